@@ -1,0 +1,569 @@
+// k_pair.hip -- neighbour lists, PSE heat, and the fused pair + particle update pass.
+//
+// Neighbour pass (replaces the candidate loops of src/interactions_gpu.cu:89-150,227-336):
+//   for particle i in cell (gi, gj, gk) the reference visits cells (ii, jj, kk) of the 5x5x5
+//   stencil in ascending order and, inside a cell, particles in sorted order.  Cells along kk are
+//   contiguous in the sorted order, so each (ii, jj) pair is ONE contiguous particle range
+//   [off[c(ii,jj,klo)], off[c(ii,jj,khi)+1]); the 25 ranges in (ii, jj) order reproduce the
+//   reference's visiting order exactly.  Candidates are kept if not blanked and inside the
+//   cubic-spline support (1e-8 <= r^2 < 4 h_i^2, src/kernels.cuh:41), the only pairs whose terms
+//   are non-zero.  The PSE heat sum (no cutoff, blanked j included, src/interactions_gpu.cu:
+//   124-147) runs in the same candidate loop.
+//
+// Pair pass (src/interactions_gpu.cu:152-471 + src/actions_gpu.cu:134-300,302-331,371-449 +
+// src/leap_frog.cu:72-100): per particle, CSPM matrix, velocity gradient, stress divergence with
+// Gray artificial stress, Monaghan viscosity, XSPH, Brookshaw heat over the neighbour list, then
+// in registers: Jaumann rate, continuity, momentum, advection, NaN/Inf invalidation, leap-frog
+// correction and the Johnson-Cook radial return (FP64).
+//
+// Precision: FP64 evaluates the reference expressions in the reference order.  FP32 computes
+// the relative position in FP64, the pair arithmetic in FP32 and accumulates every neighbour sum
+// with Kahan compensation; the particle update stays FP64.
+#include "mfg_device.hpp"
+
+namespace mfg {
+
+constexpr int BSP = 256;
+constexpr double PI_D = 3.14159265358979323846;
+constexpr double INV_PI_D = 0.31830988618379067154;
+
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(BSP)
+    k_neighbours(unsigned n_init, GridInfo g, const int *off, const int *key_sorted,
+                 const double4 *nl4, const double2 *pse2, const double *h, const uint32_t *flags,
+                 Consts c, int *nbr, int stride, int cap, int *count, double *T_t, Counters *cnt,
+                 int want_pse) {
+  __shared__ unsigned long long red[2][BSP / 64];
+  const unsigned i = blockIdx.x * BSP + threadIdx.x;
+  const unsigned N = cnt->N;
+  unsigned long long n_cand = 0, n_pair = 0;
+  if (i < N && i < n_init) {
+  const double4 pi = nl4[i];
+  const bool blanked_i = pi.w != 0.;
+  const double hi = h[i];
+  const double fourh2 = 4 * hi * hi;
+  const int key = key_sorted[i];
+  const int nyz = g.ny * g.nz;
+  const int gi = key / nyz;
+  const int gj = (key - gi * nyz) / g.nz;
+  const int gk = key % g.nz;
+  const int lo_i = gi - 2 < 0 ? 0 : gi - 2, hi_i = gi + 3 > g.nx ? g.nx : gi + 3;
+  const int lo_j = gj - 2 < 0 ? 0 : gj - 2, hi_j = gj + 3 > g.ny ? g.ny : gj + 3;
+  const int lo_k = gk - 2 < 0 ? 0 : gk - 2, hi_k = gk + 3 > g.nz ? g.nz : gk + 3;
+  double Ti = 0., coef = 0., h2 = 0., Tti = 0.;
+  if (want_pse) {
+    Ti = pse2[i].x;
+    h2 = hi * hi;
+    double h4 = h2 * h2;
+    double h5 = hi * h4;
+    coef = +4. / (h5 * PI_D * sqrt_rn(PI_D));
+  }
+  int n = 0;
+  for (int ii = lo_i; ii < hi_i; ii++)
+    for (int jj = lo_j; jj < hi_j; jj++) {
+      const int cb = ii * nyz + jj * g.nz;
+      const int s = off[cb + lo_k], e = off[cb + hi_k];
+      n_cand += (unsigned long long)(e - s);
+      for (int j = s; j < e; j++) {
+        const double4 pj = nl4[j];
+        double xij = pi.x - pj.x, yij = pi.y - pj.y, zij = pi.z - pj.z;
+        double rr2 = xij * xij + yij * yij + zij * zij;
+        if (want_pse) {
+          const double2 tr = pse2[j];
+          double xx = sqrt_rn(rr2);
+          double w2 = coef * exp(-xx * xx / (h2));
+          Tti += (tr.x - Ti) * w2 * c.i_mass / tr.y;
+        }
+        if (!blanked_i && pj.w == 0. && !(rr2 >= fourh2 || rr2 < 1e-8)) {
+          if (n < cap) nbr[(size_t)n * stride + i] = j;
+          n++;
+        }
+      }
+    }
+  if (n > cap) {
+    atomicOr(&cnt->err_capacity, 1);
+    atomicMax(&cnt->max_count, n);
+    n = cap;
+  } else {
+    atomicMax(&cnt->max_count, n);
+  }
+  count[i] = n;
+  if (want_pse) T_t[i] = ((flags[i] & FL_TOOL) ? c.alpha_tool : c.alpha_wp) * Tti;
+  n_pair = (unsigned long long)n;
+  }
+  // candidate / pair totals of this pass (work accounting of the roofline figures)
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int o = 32; o > 0; o >>= 1) {
+    n_cand += __shfl_down(n_cand, o, 64);
+    n_pair += __shfl_down(n_pair, o, 64);
+  }
+  if (lane == 0) {
+    red[0][wid] = n_cand;
+    red[1][wid] = n_pair;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long a = 0, b = 0;
+    for (int w = 0; w < BSP / 64; w++) {
+      a += red[0][w];
+      b += red[1][w];
+    }
+    if (a) atomicAdd(&cnt->candidates, a);
+    if (b) atomicAdd(&cnt->pairs, b);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// particle update after the neighbour sums (FP64 for both precisions)
+struct PairSums {
+  M3 vd, sd;
+  double vt[3], xt[3], Tl;
+};
+
+__device__ __forceinline__ void particle_update(const State &st, unsigned i, uint32_t fl,
+                                                const Consts &c, const PairSums &ps,
+                                                bool brook, double thermal_alpha) {
+  const bool tool = (fl & FL_TOOL) != 0;
+  const double dt = c.dt, hdt = 0.5 * c.dt;
+  double x0[3] = {st.x[i], st.y[i], st.z[i]};
+  double v0[3] = {st.vx[i], st.vy[i], st.vz[i]};
+  double S0[6], Sto[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    S0[k] = st.S[k][i];
+    Sto[k] = st.St[k][i];
+  }
+  double vto[3] = {st.vxt[i], st.vyt[i], st.vzt[i]};
+  double rho0 = st.rho[i], T0 = st.T[i], rho_to = st.rho_t[i], T_to = st.T_t[i];
+  double nxt[3], nvt[3], nSt[6], nrho_t, nT_t;
+  if (!tool) {
+    double Sh[6], vh[3];
+#pragma unroll
+    for (int k = 0; k < 6; k++) Sh[k] = S0[k] + hdt * Sto[k];
+#pragma unroll
+    for (int k = 0; k < 3; k++) vh[k] = v0[k] + hdt * vto[k];
+    double rhoh = rho0 + hdt * rho_to;
+    // Jaumann, continuity, momentum, advection (src/actions_gpu.cu:134-300)
+    jaumann(c.a_G, ps.vd, Sh, nSt);
+    nrho_t = -rhoh * (ps.vd.m[0][0] + ps.vd.m[1][1] + ps.vd.m[2][2]);
+    const double m = c.a_mass;
+    double fc[3] = {st.fcx[i], st.fcy[i], st.fcz[i]};
+    double ft[3] = {st.ftx[i], st.fty[i], st.ftz[i]};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      nvt[a] = ps.vt[a] + (ps.sd.m[a][0] + ps.sd.m[a][1] + ps.sd.m[a][2] + fc[a] / m + ft[a] / m);
+      nxt[a] = ps.xt[a] + vh[a];
+    }
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      nvt[a] = ps.vt[a];
+      nxt[a] = ps.xt[a];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) nSt[k] = Sto[k];
+    nrho_t = rho_to;
+  }
+  nT_t = brook ? thermal_alpha * ps.Tl : T_to;
+  st.xt[i] = nxt[0]; st.yt[i] = nxt[1]; st.zt[i] = nxt[2];
+  st.vxt[i] = nvt[0]; st.vyt[i] = nvt[1]; st.vzt[i] = nvt[2];
+#pragma unroll
+  for (int k = 0; k < 6; k++) st.St[k][i] = nSt[k];
+  st.rho_t[i] = nrho_t;
+  st.T_t[i] = nT_t;
+  // debug_invalidate (src/actions_gpu.cu:302-331)
+  auto bad = [](double v) { return isnan(v) || isinf(v); };
+  bool invalid = bad(nxt[0]) || bad(nxt[1]) || bad(nxt[2]) || bad(nvt[0]) || bad(nvt[1]) ||
+                 bad(nvt[2]) || bad(nSt[0]) || bad(nSt[1]) || bad(nSt[2]) || bad(nSt[3]) ||
+                 bad(nSt[4]) || bad(nSt[5]) || bad(nrho_t) || bad(nT_t);
+  if (invalid) {
+    st.flags[i] = fl | FL_BLANKED;
+    return;
+  }
+  // correct (src/leap_frog.cu:72-100)
+  double x[3], v[3], S[6];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    x[k] = x0[k] + dt * nxt[k];
+    v[k] = v0[k] + dt * nvt[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 6; k++) S[k] = S0[k] + dt * nSt[k];
+  double rho = rho0 + dt * nrho_t;
+  double T = T0 + dt * nT_t;
+  // Johnson-Cook radial return (src/actions_gpu.cu:371-449)
+  if (c.plastic && !tool) {
+    double sxx = S[0], syy = S[1], szz = S[2], sxy = S[3], syz = S[4], szx = S[5];
+    double nrm = sqrt_rn(sxx * sxx + syy * syy + szz * szz +
+                            2 * (sxy * sxy + syz * syz + szx * szx));
+    double ep = st.eps_pl[i], epd = st.eps_pl_dot[i];
+    double Tc = T;
+    if (c.jc_clamp != 0.) Tc = (Tc > c.jc_Tmelt) ? c.jc_Tmelt - 1 : Tc;
+    double svm = sqrt_rn(3.0 / 2.0) * nrm;
+    double sy = sigma_yield(c, ep, epd, Tc);
+    if (svm < sy) {
+      st.eps_pl_dot[i] = 0.;
+    } else {
+      bool failed = false;
+      double dl = solve_secant(c, fmax(epd * dt * sqrt_rn(2. / 3.), 1e-8), 1e-6, nrm, ep, Tc,
+                               dt, c.a_G, failed);
+      double ep_new, epd_new;
+      if (c.single_sqrt) {
+        float f = sqrt_rn((float)(2.0 / 3.0)) * fmaxf((float)dl, 0.f);
+        ep_new = ep + (double)f;
+        epd_new = (double)f / dt;
+      } else {
+        double f = sqrt_rn(2.0 / 3.0) * fmax(dl, 0.);
+        ep_new = ep + f;
+        epd_new = sqrt_rn(2.0 / 3.0) * fmax(dl, 0.) / dt;
+      }
+      st.eps_pl[i] = ep_new;
+      st.eps_pl_dot[i] = epd_new;
+      const double mu = c.a_G;
+      S[0] = sxx - sxx / nrm * dl * 2. * mu;
+      S[1] = syy - syy / nrm * dl * 2. * mu;
+      S[2] = szz - szz / nrm * dl * 2. * mu;
+      S[3] = sxy - sxy / nrm * dl * 2. * mu;
+      S[4] = syz - syz / nrm * dl * 2. * mu;
+      S[5] = szx - szx / nrm * dl * 2. * mu;
+      if (c.twp_tq != 0.) {
+        double sy2 = sigma_yield(c, ep_new, epd_new, Tc);
+        T += c.twp_tq / (c.twp_cp * rho) * (ep_new - ep) * sy2;
+      }
+    }
+  }
+  st.x[i] = x[0]; st.y[i] = x[1]; st.z[i] = x[2];
+  st.vx[i] = v[0]; st.vy[i] = v[1]; st.vz[i] = v[2];
+#pragma unroll
+  for (int k = 0; k < 6; k++) st.S[k][i] = S[k];
+  st.rho[i] = rho;
+  st.T[i] = T;
+}
+
+// ---------------------------------------------------------------------------------------------
+// FP64 pair pass
+__global__ void __launch_bounds__(BSP)
+    k_pair64(unsigned n_init, State st, const Rec64 *__restrict__ rec, const int *__restrict__ nbr,
+             int stride, const int *__restrict__ count, Consts c, const Counters *cnt) {
+  const unsigned i = blockIdx.x * BSP + threadIdx.x;
+  const unsigned N = cnt->N;
+  if (i >= N || i >= n_init) return;
+  const uint32_t fl = st.flags[i];
+  if (fl & FL_BLANKED) return;
+  const Rec64 ri = rec[i];
+  const bool tool_i = ri.tool == 1.;
+  const bool tool_nz = (fl & FL_TOOL_NZ) != 0;
+  const double thermal_alpha = tool_nz ? c.alpha_tool : c.alpha_wp;
+  const bool brook = c.heat == MFG_HEAT_BROOKSHAW && thermal_alpha != 0.;
+  const double hi = ri.h;
+  const double h1 = 1 / hi;
+  const double fac = (INV_PI_D)*h1 * h1 * h1;
+  const int n = count[i];
+  const int single = c.single_sqrt;
+  const double mass = c.i_mass;
+
+  M3 invB;
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) invB.m[a][b] = (a == b) ? 1. : 0.;
+  if (c.cspm) {
+    M3 B;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) B.m[a][b] = 0.;
+    if (!tool_nz) {
+      for (int k = 0; k < n; k++) {
+        const int j = nbr[(size_t)k * stride + i];
+        const double4 pj = *(const double4 *)&rec[j].x;
+        const double volj = rec[j].vol, tj = rec[j].tool;
+        if (tj == 1.) continue;
+        double xij = ri.x - pj.x, yij = ri.y - pj.y, zij = ri.z - pj.z;
+        double rr2 = xij * xij + yij * yij + zij * zij;
+        double w, wx, wy, wz;
+        cubic_spline64(xij, yij, zij, rr2, h1, fac, single, w, wx, wy, wz);
+        B.m[0][0] -= volj * xij * wx; B.m[1][0] -= volj * xij * wy; B.m[2][0] -= volj * xij * wz;
+        B.m[0][1] -= volj * yij * wx; B.m[1][1] -= volj * yij * wy; B.m[2][1] -= volj * yij * wz;
+        B.m[0][2] -= volj * zij * wx; B.m[1][2] -= volj * zij * wy; B.m[2][2] -= volj * zij * wz;
+      }
+    }
+    if (det3(B) > 1e-8) inv3(B, invB);
+  }
+
+  PairSums ps;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      ps.vd.m[a][b] = 0.;
+      ps.sd.m[a][b] = 0.;
+    }
+    ps.vt[a] = 0.;
+    ps.xt[a] = 0.;
+  }
+  ps.Tl = 0.;
+  const double Ti = brook ? ri.T : 0.;
+  const double wdeltap = c.wdeltap, eps = c.xspheps;
+  for (int k = 0; k < n; k++) {
+    const int j = nbr[(size_t)k * stride + i];
+    const Rec64 rj = rec[j];
+    double xij = ri.x - rj.x, yij = ri.y - rj.y, zij = ri.z - rj.z;
+    double rr2 = xij * xij + yij * yij + zij * zij;
+    double w, wx0, wy0, wz0;
+    cubic_spline64(xij, yij, zij, rr2, h1, fac, single, w, wx0, wy0, wz0);
+    double w_x, w_y, w_z;
+    if (!c.cspm) {
+      w_x = wx0; w_y = wy0; w_z = wz0;
+    } else {
+      w_x = (wx0 * invB.m[0][0] + wy0 * invB.m[1][0] + wz0 * invB.m[2][0]);
+      w_y = (wx0 * invB.m[0][1] + wy0 * invB.m[1][1] + wz0 * invB.m[2][1]);
+      w_z = (wx0 * invB.m[0][2] + wy0 * invB.m[1][2] + wz0 * invB.m[2][2]);
+    }
+    if (!tool_i && !(rj.tool == 1.)) {
+      const double volj = rj.vol;
+      double dvx = rj.vx - ri.vx, dvy = rj.vy - ri.vy, dvz = rj.vz - ri.vz;
+      ps.vd.m[0][0] += dvx * w_x * volj; ps.vd.m[0][1] += dvx * w_y * volj; ps.vd.m[0][2] += dvx * w_z * volj;
+      ps.vd.m[1][0] += dvy * w_x * volj; ps.vd.m[1][1] += dvy * w_y * volj; ps.vd.m[1][2] += dvy * w_z * volj;
+      ps.vd.m[2][0] += dvz * w_x * volj; ps.vd.m[2][1] += dvz * w_y * volj; ps.vd.m[2][2] += dvz * w_z * volj;
+      double Rxx = 0., Ryy = 0., Rzz = 0., Rxy = 0., Rxz = 0., Ryz = 0.;
+      if (wdeltap > 0) {
+        double fab = w / wdeltap;
+        fab *= fab;
+        fab *= fab;
+        Rxx = fab * (ri.R[0] + rj.R[0]);
+        Rxy = fab * (ri.R[3] + rj.R[3]);
+        Ryy = fab * (ri.R[1] + rj.R[1]);
+        Rxz = fab * (ri.R[5] + rj.R[5]);
+        Ryz = fab * (ri.R[4] + rj.R[4]);
+        Rzz = fab * (ri.R[2] + rj.R[2]);
+      }
+      ps.sd.m[0][0] += mass * (ri.A[0] + rj.A[0] + Rxx) * w_x;
+      ps.sd.m[0][1] += mass * (ri.A[3] + rj.A[3] + Rxy) * w_y;
+      ps.sd.m[0][2] += mass * (ri.A[5] + rj.A[5] + Rxz) * w_z;
+      ps.sd.m[1][0] += mass * (ri.A[3] + rj.A[3] + Rxy) * w_x;
+      ps.sd.m[1][1] += mass * (ri.A[1] + rj.A[1] + Ryy) * w_y;
+      ps.sd.m[1][2] += mass * (ri.A[4] + rj.A[4] + Ryz) * w_z;
+      ps.sd.m[2][0] += mass * (ri.A[5] + rj.A[5] + Rxz) * w_x;
+      ps.sd.m[2][1] += mass * (ri.A[4] + rj.A[4] + Ryz) * w_y;
+      ps.sd.m[2][2] += mass * (ri.A[2] + rj.A[2] + Rzz) * w_z;
+      double vijx = ri.vx - rj.vx, vijy = ri.vy - rj.vy, vijz = ri.vz - rj.vz;
+      double vijposij = vijx * xij + vijy * yij + vijz * zij;
+      double rhoij = 0.5 * (ri.rho + rj.rho);
+      if (vijposij < 0.) {
+        double cij = 0.5 * (ri.c + rj.c);
+        double hij = 0.5 * (hi + rj.h);
+        double r2ij = xij * xij + yij * yij + zij * zij;
+        double muij = (hij * vijposij) / (r2ij + c.v_eta * c.v_eta * hij * hij);
+        double piij = (-c.v_alpha * cij * muij + c.v_beta * muij * muij) / rhoij;
+        ps.vt[0] += -mass * piij * w_x;
+        ps.vt[1] += -mass * piij * w_y;
+        ps.vt[2] += -mass * piij * w_z;
+      }
+      ps.xt[0] += -eps * w * mass / rhoij * vijx;
+      ps.xt[1] += -eps * w * mass / rhoij * vijy;
+      ps.xt[2] += -eps * w * mass / rhoij * vijz;
+    }
+    if (brook) {
+      double rij = sqrt_rn(xij * xij + yij * yij + zij * zij);
+      if (rij > 1e-8) {
+        double ex = xij / rij, ey = yij / rij, ez = zij / rij;
+        double rij1 = 1. / rij;
+        ps.Tl += 2.0 * (rj.vol) * (Ti - rj.T) * rij1 * (ex * w_x + ey * w_y + ez * w_z);
+      }
+    }
+  }
+  particle_update(st, i, fl, c, ps, brook, thermal_alpha);
+}
+
+// ---------------------------------------------------------------------------------------------
+// FP32 pair pass with Kahan-compensated neighbour sums
+struct KF {
+  float s, e;
+  __device__ __forceinline__ void add(float v) {
+    float y = v - e;
+    float t = s + y;
+    e = (t - s) - y;
+    s = t;
+  }
+  __device__ __forceinline__ double val() const { return (double)s - (double)e; }
+};
+
+__global__ void __launch_bounds__(BSP)
+    k_pair32(unsigned n_init, State st, const Rec32 *__restrict__ rec, const int *__restrict__ nbr,
+             int stride, const int *__restrict__ count, Consts c, const Counters *cnt) {
+  const unsigned i = blockIdx.x * BSP + threadIdx.x;
+  const unsigned N = cnt->N;
+  if (i >= N || i >= n_init) return;
+  const uint32_t fl = st.flags[i];
+  if (fl & FL_BLANKED) return;
+  const Rec32 ri = rec[i];
+  const bool tool_i = ri.tool == 1.f;
+  const bool tool_nz = (fl & FL_TOOL_NZ) != 0;
+  const double thermal_alpha = tool_nz ? c.alpha_tool : c.alpha_wp;
+  const bool brook = c.heat == MFG_HEAT_BROOKSHAW && thermal_alpha != 0.;
+  const float hi = ri.h;
+  const float h1 = __builtin_amdgcn_rcpf(hi);
+  const float fac = (float)INV_PI_D * h1 * h1 * h1;
+  const int n = count[i];
+  const float mass = (float)c.i_mass;
+
+  float iB[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};  // invB[col][row] flattened
+  if (c.cspm) {
+    KF B[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) B[q] = KF{0.f, 0.f};
+    if (!tool_nz) {
+      for (int k = 0; k < n; k++) {
+        const int j = nbr[(size_t)k * stride + i];
+        const Rec32 &rj = rec[j];
+        if (rj.tool == 1.f) continue;
+        float xij = (float)(ri.x - rj.x), yij = (float)(ri.y - rj.y), zij = (float)(ri.z - rj.z);
+        float rr2 = xij * xij + yij * yij + zij * zij;
+        float rad = __builtin_amdgcn_sqrtf(rr2);
+        float rinv = __builtin_amdgcn_rcpf(rad);
+        float q = rad * h1;
+        float der = (q > 1.f) ? -0.75f * (2.f - q) * (2.f - q) * h1 * rinv
+                              : -3.0f * q * (1.f - 0.75f * q) * h1 * rinv;
+        float dw = der * fac * rj.vol;
+        float gx = dw * xij, gy = dw * yij, gz = dw * zij;
+        B[0].add(-xij * gx); B[3].add(-xij * gy); B[6].add(-xij * gz);
+        B[1].add(-yij * gx); B[4].add(-yij * gy); B[7].add(-yij * gz);
+        B[2].add(-zij * gx); B[5].add(-zij * gy); B[8].add(-zij * gz);
+      }
+    }
+    M3 Bd, inv;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) Bd.m[a][b] = B[3 * a + b].val();
+    if (det3(Bd) > 1e-8) {
+      inv3(Bd, inv);
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) iB[3 * a + b] = (float)inv.m[a][b];
+    }
+  }
+
+  KF vd[9], sd[9], vt[3], xt[3], Tl;
+#pragma unroll
+  for (int q = 0; q < 9; q++) {
+    vd[q] = KF{0.f, 0.f};
+    sd[q] = KF{0.f, 0.f};
+  }
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    vt[a] = KF{0.f, 0.f};
+    xt[a] = KF{0.f, 0.f};
+  }
+  Tl = KF{0.f, 0.f};
+  const float Ti = brook ? ri.T : 0.f;
+  const float inv_wdp = c.wdeltap > 0 ? (float)(1.0 / c.wdeltap) : 0.f;
+  const float eps = (float)c.xspheps, va = (float)c.v_alpha, vb = (float)c.v_beta;
+  const float ve2 = (float)(c.v_eta * c.v_eta);
+  for (int k = 0; k < n; k++) {
+    const int j = nbr[(size_t)k * stride + i];
+    const Rec32 rj = rec[j];
+    float xij = (float)(ri.x - rj.x), yij = (float)(ri.y - rj.y), zij = (float)(ri.z - rj.z);
+    float rr2 = xij * xij + yij * yij + zij * zij;
+    float rad = __builtin_amdgcn_sqrtf(rr2);
+    float rinv = __builtin_amdgcn_rcpf(rad);
+    float q = rad * h1;
+    float val, der;
+    if (q > 1.f) {
+      float a = 2.f - q;
+      val = 0.25f * a * a * a;
+      der = -0.75f * a * a * h1 * rinv;
+    } else {
+      val = 1.f - 1.5f * q * q * (1.f - 0.5f * q);
+      der = -3.0f * q * (1.f - 0.75f * q) * h1 * rinv;
+    }
+    float w = val * fac;
+    float d = der * fac;
+    float wx0 = d * xij, wy0 = d * yij, wz0 = d * zij;
+    float w_x = wx0 * iB[0] + wy0 * iB[3] + wz0 * iB[6];
+    float w_y = wx0 * iB[1] + wy0 * iB[4] + wz0 * iB[7];
+    float w_z = wx0 * iB[2] + wy0 * iB[5] + wz0 * iB[8];
+    if (!tool_i && !(rj.tool == 1.f)) {
+      const float volj = rj.vol;
+      float dvx = (rj.vx - ri.vx) * volj, dvy = (rj.vy - ri.vy) * volj, dvz = (rj.vz - ri.vz) * volj;
+      vd[0].add(dvx * w_x); vd[1].add(dvx * w_y); vd[2].add(dvx * w_z);
+      vd[3].add(dvy * w_x); vd[4].add(dvy * w_y); vd[5].add(dvy * w_z);
+      vd[6].add(dvz * w_x); vd[7].add(dvz * w_y); vd[8].add(dvz * w_z);
+      float fab = w * inv_wdp;
+      fab *= fab;
+      fab *= fab;
+      float Rxx = fab * (ri.R[0] + rj.R[0]), Ryy = fab * (ri.R[1] + rj.R[1]);
+      float Rzz = fab * (ri.R[2] + rj.R[2]), Rxy = fab * (ri.R[3] + rj.R[3]);
+      float Ryz = fab * (ri.R[4] + rj.R[4]), Rxz = fab * (ri.R[5] + rj.R[5]);
+      float mx = mass * w_x, my = mass * w_y, mz = mass * w_z;
+      float Txx = ri.A[0] + rj.A[0] + Rxx, Tyy = ri.A[1] + rj.A[1] + Ryy, Tzz = ri.A[2] + rj.A[2] + Rzz;
+      float Txy = ri.A[3] + rj.A[3] + Rxy, Tyz = ri.A[4] + rj.A[4] + Ryz, Txz = ri.A[5] + rj.A[5] + Rxz;
+      sd[0].add(Txx * mx); sd[1].add(Txy * my); sd[2].add(Txz * mz);
+      sd[3].add(Txy * mx); sd[4].add(Tyy * my); sd[5].add(Tyz * mz);
+      sd[6].add(Txz * mx); sd[7].add(Tyz * my); sd[8].add(Tzz * mz);
+      float vijx = ri.vx - rj.vx, vijy = ri.vy - rj.vy, vijz = ri.vz - rj.vz;
+      float vijposij = vijx * xij + vijy * yij + vijz * zij;
+      float rhoij_inv = __builtin_amdgcn_rcpf(0.5f * (ri.rho + rj.rho));
+      if (vijposij < 0.f) {
+        float cij = 0.5f * (ri.c + rj.c);
+        float hij = 0.5f * (hi + rj.h);
+        float muij = (hij * vijposij) * __builtin_amdgcn_rcpf(rr2 + ve2 * hij * hij);
+        float piij = (-va * cij * muij + vb * muij * muij) * rhoij_inv;
+        float f = -mass * piij;
+        vt[0].add(f * w_x);
+        vt[1].add(f * w_y);
+        vt[2].add(f * w_z);
+      }
+      float g = -eps * w * mass * rhoij_inv;
+      xt[0].add(g * vijx);
+      xt[1].add(g * vijy);
+      xt[2].add(g * vijz);
+    }
+    if (brook && rad > 1e-8f) {
+      float e_w = (xij * w_x + yij * w_y + zij * w_z) * rinv;
+      Tl.add(2.0f * rj.vol * (Ti - rj.T) * rinv * e_w);
+    }
+  }
+  PairSums ps;
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      ps.vd.m[a][b] = vd[3 * a + b].val();
+      ps.sd.m[a][b] = sd[3 * a + b].val();
+    }
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    ps.vt[a] = vt[a].val();
+    ps.xt[a] = xt[a].val();
+  }
+  ps.Tl = Tl.val();
+  particle_update(st, i, fl, c, ps, brook, thermal_alpha);
+}
+
+// ---------------------------------------------------------------------------------------------
+static inline unsigned nb(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+void launch_neighbours(hipStream_t s, unsigned n_init, const GridInfo &g, const int *offsets,
+                       const int *key_sorted, const double4 *nl4, const double2 *pse2,
+                       const double *h, const uint32_t *flags, const Consts &c, int *nbr,
+                       int nl_stride, int cap, int *count, double *T_t, Counters *cnt,
+                       int want_pse) {
+  hipLaunchKernelGGL(k_neighbours, dim3(nb(n_init, BSP)), dim3(BSP), 0, s, n_init, g, offsets,
+                     key_sorted, nl4, pse2, h, flags, c, nbr, nl_stride, cap, count, T_t, cnt,
+                     want_pse);
+}
+
+void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *rec, const int *nbr,
+                 int nl_stride, const int *count, const Consts &c, const Counters *cnt) {
+  if (c.precision == MFG_FP64)
+    hipLaunchKernelGGL(k_pair64, dim3(nb(n_init, BSP)), dim3(BSP), 0, s, n_init, st,
+                       (const Rec64 *)rec, nbr, nl_stride, count, c, cnt);
+  else
+    hipLaunchKernelGGL(k_pair32, dim3(nb(n_init, BSP)), dim3(BSP), 0, s, n_init, st,
+                       (const Rec32 *)rec, nbr, nl_stride, count, c, cnt);
+}
+
+}  // namespace mfg

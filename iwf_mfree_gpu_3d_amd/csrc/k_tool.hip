@@ -1,0 +1,535 @@
+// k_tool.hip -- tool mesh on the GPU: triangle binning, motion, contact, boundary conditions.
+//
+//   triangle binning          grid_cpu_tri::get_cells + mesh_triangle::cover_aabb
+//                             (src/grid_cpu_tri.cpp:96-175, src/surface_tri_mesh.cpp:151-278),
+//                             done here with a counting sort over cells on the device
+//   normal orientation        tool_3d ctor (src/tool_3d.cpp:318-358)
+//   do_update_tool            src/tool_3d_gpu.cu:74-99
+//   contact + boundary pass   do_reset_contact_forces, do_compute_contact_forces[_n2],
+//                             do_material_fric_heat_gen, do_boundary_conditions[_thermal],
+//                             do_move_tool_particles  (src/leap_frog.cu:102-119,
+//                             src/tool_3d_gpu.cu:282-681, src/actions_gpu.cu:55-68,192-220,451-487)
+//   tool force recording      per-block partial sums reduced in a fixed order (the reference
+//                             uses float atomics, src/tool_3d_gpu.cu:579-581)
+#include <cfloat>
+
+#include "mfg_device.hpp"
+
+namespace mfg {
+
+constexpr int BST = 256;
+
+// ---- triangle / box overlap ------------------------------------------------------------------
+__device__ __forceinline__ bool axis_reject(double pa, double pb, double rad) {
+  double mn = pa < pb ? pa : pb, mx = pa < pb ? pb : pa;
+  return (mn > rad || mx < -rad);
+}
+
+__device__ bool plane_box_overlap(V3 nrm, V3 vert, V3 mb) {
+  double n[3] = {nrm.x, nrm.y, nrm.z}, v[3] = {vert.x, vert.y, vert.z}, m[3] = {mb.x, mb.y, mb.z};
+  double vmin[3], vmax[3];
+  for (int q = 0; q < 3; q++) {
+    if (n[q] > 0.0) {
+      vmin[q] = -m[q] - v[q];
+      vmax[q] = m[q] - v[q];
+    } else {
+      vmin[q] = m[q] - v[q];
+      vmax[q] = -m[q] - v[q];
+    }
+  }
+  if (vdot(nrm, v3(vmin[0], vmin[1], vmin[2])) > 0.0) return false;
+  if (vdot(nrm, v3(vmax[0], vmax[1], vmax[2])) >= 0.0) return true;
+  return false;
+}
+
+__device__ bool tri_box(const Tri &t, V3 bmin, V3 bmax) {
+  V3 c = vscale(0.5, vadd(bmin, bmax));
+  V3 hs = vscale(0.5, vsub(bmax, bmin));
+  V3 p1 = vload(t.p[0]), p2 = vload(t.p[1]), p3 = vload(t.p[2]);
+  V3 v0 = vsub(p1, c), v1 = vsub(p2, c), v2 = vsub(p3, c);
+  V3 e0 = vsub(p2, p1), e1 = vsub(p3, p2), e2 = vsub(p1, p3);
+  double fx = fabs(e0.x), fy = fabs(e0.y), fz = fabs(e0.z);
+  if (axis_reject(e0.z * v0.y - e0.y * v0.z, e0.z * v2.y - e0.y * v2.z, fz * hs.y + fy * hs.z)) return false;
+  if (axis_reject(-e0.z * v0.x + e0.x * v0.z, -e0.z * v2.x + e0.x * v2.z, fz * hs.x + fx * hs.z)) return false;
+  if (axis_reject(e0.y * v1.x - e0.x * v1.y, e0.y * v2.x - e0.x * v2.y, fy * hs.x + fx * hs.y)) return false;
+  // the second and third edge magnitudes are taken in single precision as the reference does
+  fx = fabsf((float)e1.x); fy = fabsf((float)e1.y); fz = fabsf((float)e1.z);
+  if (axis_reject(e1.z * v0.y - e1.y * v0.z, e1.z * v2.y - e1.y * v2.z, fz * hs.y + fy * hs.z)) return false;
+  if (axis_reject(-e1.z * v0.x + e1.x * v0.z, -e1.z * v2.x + e1.x * v2.z, fz * hs.x + fx * hs.z)) return false;
+  if (axis_reject(e1.y * v0.x - e1.x * v0.y, e1.y * v1.x - e1.x * v1.y, fy * hs.x + fx * hs.y)) return false;
+  fx = fabsf((float)e2.x); fy = fabsf((float)e2.y); fz = fabsf((float)e2.z);
+  if (axis_reject(e2.z * v0.y - e2.y * v0.z, e2.z * v1.y - e2.y * v1.z, fz * hs.y + fy * hs.z)) return false;
+  if (axis_reject(-e2.z * v0.x + e2.x * v0.z, -e2.z * v1.x + e2.x * v1.z, fz * hs.x + fx * hs.z)) return false;
+  if (axis_reject(e2.y * v1.x - e2.x * v1.y, e2.y * v2.x - e2.x * v2.y, fy * hs.x + fx * hs.y)) return false;
+  double mn, mx;
+  mn = fmin(v0.x, fmin(v1.x, v2.x)); mx = fmax(v0.x, fmax(v1.x, v2.x));
+  if (mn > hs.x || mx < -hs.x) return false;
+  mn = fmin(v0.y, fmin(v1.y, v2.y)); mx = fmax(v0.y, fmax(v1.y, v2.y));
+  if (mn > hs.y || mx < -hs.y) return false;
+  mn = fmin(v0.z, fmin(v1.z, v2.z)); mx = fmax(v0.z, fmax(v1.z, v2.z));
+  if (mn > hs.z || mx < -hs.z) return false;
+  return plane_box_overlap(vcross(e0, e1), v0, hs);
+}
+
+struct TriGrid {
+  double bbmin[3];
+  double dx;
+  int nx, ny, nz;
+};
+
+__device__ __forceinline__ void tri_cell_range(const Tri &t, const TriGrid &g, unsigned lo[3],
+                                               unsigned hi[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    double mn = fmin(fmin(t.p[0][k], t.p[1][k]), t.p[2][k]);
+    double mx = fmax(fmax(t.p[0][k], t.p[1][k]), t.p[2][k]);
+    lo[k] = (unsigned)((mn - g.bbmin[k]) / g.dx);
+    hi[k] = (unsigned)((mx - g.bbmin[k]) / g.dx);
+  }
+  // the grid is padded around every triangle; the clamp only guards rounding at the last cell
+  const unsigned n[3] = {(unsigned)g.nx, (unsigned)g.ny, (unsigned)g.nz};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    if (hi[k] >= n[k]) hi[k] = n[k] - 1;
+    if (lo[k] > hi[k]) lo[k] = hi[k];
+  }
+}
+
+__global__ void k_tri_bin(const Tri *tris, int n, TriGrid g, int *count, int *fill, int *out) {
+  int t = blockIdx.x * BST + threadIdx.x;
+  if (t >= n) return;
+  const Tri tr = tris[t];
+  unsigned lo[3], hi[3];
+  tri_cell_range(tr, g, lo, hi);
+  for (unsigned a = lo[0]; a <= hi[0]; a++)
+    for (unsigned b = lo[1]; b <= hi[1]; b++)
+      for (unsigned cc = lo[2]; cc <= hi[2]; cc++) {
+        unsigned lin = a * (g.ny) * (g.nz) + b * (g.nz) + cc;
+        V3 bl = v3(g.bbmin[0] + a * g.dx, g.bbmin[1] + b * g.dx, g.bbmin[2] + cc * g.dx);
+        V3 bh = v3(g.bbmin[0] + (a + 1) * g.dx, g.bbmin[1] + (b + 1) * g.dx,
+                   g.bbmin[2] + (cc + 1) * g.dx);
+        if (!tri_box(tr, bl, bh)) continue;
+        if (!out) atomicAdd(&count[lin], 1);
+        else out[count[lin] + atomicAdd(&fill[lin], 1)] = t;
+      }
+}
+
+// exclusive scan of the per-cell triangle counts (tool grids are small: one block)
+__global__ void __launch_bounds__(1024) k_tri_scan(int *count, int n) {
+  __shared__ int part[1024];
+  int per = (n + 1023) / 1024;
+  int beg = threadIdx.x * per, end = min(beg + per, n);
+  int s = 0;
+  for (int k = beg; k < end; k++) s += count[k];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int k = 0; k < 1024; k++) {
+      int v = part[k];
+      part[k] = acc;
+      acc += v;
+    }
+    count[n] = acc;
+  }
+  __syncthreads();
+  int off = part[threadIdx.x];
+  for (int k = beg; k < end; k++) {
+    int v = count[k];
+    count[k] = off;
+    off += v;
+  }
+}
+
+__global__ void k_tri_cell_sort(const int *off, int n, int *idx) {
+  int c = blockIdx.x * BST + threadIdx.x;
+  if (c >= n) return;
+  int s = off[c], e = off[c + 1];
+  for (int a = s + 1; a < e; a++) {
+    int v = idx[a], b = a - 1;
+    while (b >= s && idx[b] > v) {
+      idx[b + 1] = idx[b];
+      b--;
+    }
+    idx[b + 1] = v;
+  }
+}
+
+__global__ void k_gather_tris(const Tri *src, const int *idx, int n, Tri *dst) {
+  int k = blockIdx.x * BST + threadIdx.x;
+  if (k < n) dst[k] = src[idx[k]];
+}
+
+// ---- inside tests -------------------------------------------------------------------------
+// closest point by expanding shells of the triangle grid (src/tool_3d_gpu.cu:282-380);
+// returns true if inside by the normal test
+__device__ bool contact_by_normal(const Tri *tris, const int *cells, int nx, int ny, int nz,
+                                  int ix, int iy, int iz, V3 qp, V3 &cp, V3 &nrm) {
+  int box = ix * ny * nz + iy * nz + iz;
+  double min_dist = DBL_MAX;
+  V3 bc = v3(0, 0, 0), bn = v3(0, 0, 0);
+  bool all_empty = (cells[box] == cells[box + 1]);
+  for (int p = cells[box]; p < cells[box + 1]; p++) {
+    V3 c, n;
+    closest_point(tris[p], qp, c, n);
+    double d = dist2(c, qp);
+    if (d < min_dist) {
+      min_dist = d;
+      bc = c;
+      bn = n;
+    }
+  }
+  int offset = 1;
+  bool all_done = false;
+  while (true) {
+    int lo_i = ix - offset < 0 ? 0 : ix - offset, hi_i = ix + offset + 1 > nx ? nx : ix + offset + 1;
+    int lo_j = iy - offset < 0 ? 0 : iy - offset, hi_j = iy + offset + 1 > ny ? ny : iy + offset + 1;
+    int lo_k = iz - offset < 0 ? 0 : iz - offset, hi_k = iz + offset + 1 > nz ? nz : iz + offset + 1;
+    for (int a = lo_i; a < hi_i; a++)
+      for (int b = lo_j; b < hi_j; b++)
+        for (int cc = lo_k; cc < hi_k; cc++) {
+          bool on = (a == lo_i || a == hi_i - 1) || (b == lo_j || b == hi_j - 1) ||
+                    (cc == lo_k || cc == hi_k - 1);
+          if (!on) continue;
+          int bi = a * ny * nz + b * nz + cc;
+          for (int p = cells[bi]; p < cells[bi + 1]; p++) {
+            all_empty = false;
+            V3 c, n;
+            closest_point(tris[p], qp, c, n);
+            double d = dist2(c, qp);
+            if (d < min_dist) {
+              min_dist = d;
+              bc = c;
+              bn = n;
+            }
+          }
+        }
+    if (all_done) break;
+    if (!all_empty) {
+      offset += 1;
+      all_done = true;
+      continue;
+    }
+    offset += 1;
+    // every shell beyond the grid is empty: the search cannot find a triangle
+    if (offset > nx + ny + nz) break;
+  }
+  cp = bc;
+  nrm = bn;
+  V3 diff = vsub(bc, qp);
+  return !(vdot(diff, bn) > 0.);
+}
+
+// +z ray parity inside the particle's own box column (src/tool_3d_gpu.cu:382-444)
+__device__ bool contact_by_ray(const Tri *tris, const int *cells, int ny, int nz, int ix, int iy,
+                               int iz, V3 qp, const double bbmin[3], double dx) {
+  const V3 dir = v3(0., 0., 1.);
+  unsigned n_inter = 0;
+  for (; iz < nz; ++iz) {
+    unsigned bi = ix * ny * nz + iy * nz + iz;
+    for (int p = cells[bi]; p < cells[bi + 1]; p++) {
+      double t;
+      if (ray_hit(tris[p], qp, dir, t)) {
+        V3 ip = vadd(qp, vscale(t, dir));
+        double lo_x = bbmin[0] + ix * dx, lo_y = bbmin[1] + iy * dx, lo_z = bbmin[2] + iz * dx;
+        double hi_x = bbmin[0] + (ix + 1) * dx, hi_y = bbmin[1] + (iy + 1) * dx,
+               hi_z = bbmin[2] + (iz + 1) * dx;
+        if (lo_x <= ip.x && ip.x < hi_x && lo_y <= ip.y && ip.y < hi_y && lo_z <= ip.z && ip.z < hi_z)
+          n_inter++;
+      }
+    }
+  }
+  return (n_inter % 2) != 0;
+}
+
+// normal orientation test of the tool_3d constructor (host unhash: unsigned casts)
+__global__ void k_orient(const Tri *boxed, const int *cells, int nx, int ny, int nz, double b0,
+                         double b1, double b2, double dx, double q0, double q1, double q2,
+                         int *inside) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  unsigned ix = (unsigned)((q0 - b0) / dx), iy = (unsigned)((q1 - b1) / dx),
+           iz = (unsigned)((q2 - b2) / dx);
+  // the constructed outside point can fall one cell past the grid; clamp to the last cell
+  if (ix >= (unsigned)nx) ix = nx - 1;
+  if (iy >= (unsigned)ny) iy = ny - 1;
+  if (iz >= (unsigned)nz) iz = nz - 1;
+  V3 cp, n;
+  *inside = contact_by_normal(boxed, cells, nx, ny, nz, (int)ix, (int)iy, (int)iz, v3(q0, q1, q2),
+                              cp, n)
+                ? 1
+                : 0;
+}
+
+__global__ void k_flip(Tri *tris, int n) {
+  int t = blockIdx.x * BST + threadIdx.x;
+  if (t >= n) return;
+  Tri &r = tris[t];
+  for (int k = 0; k < 3; k++) {
+    r.n[k] = -r.n[k];
+    for (int q = 0; q < 3; q++) {
+      r.np[q][k] = -r.np[q][k];
+      r.ne[q][k] = -r.ne[q][k];
+    }
+  }
+}
+
+__global__ void k_tool_update(Tri *tris, const double *p_init, int n, double t, double v0,
+                              double v1, double v2) {
+  int k = blockIdx.x * BST + threadIdx.x;
+  if (k >= n) return;
+  const double v[3] = {v0, v1, v2};
+#pragma unroll
+  for (int q = 0; q < 3; q++)
+#pragma unroll
+    for (int a = 0; a < 3; a++) tris[k].p[q][a] = p_init[9 * k + 3 * q + a] + v[a] * t;
+}
+
+// ---- contact, friction heat, boundary conditions, tool particle motion ------------------------
+__global__ void __launch_bounds__(BST)
+    k_contact_bc(State st, unsigned n_init, ToolSet ts, Consts c, const Counters *cnt,
+                 float *partials) {
+  __shared__ float red[MAX_TOOLS * 3][BST / 64];
+  const unsigned i = blockIdx.x * BST + threadIdx.x;
+  const unsigned N = cnt->N;
+  float frec[MAX_TOOLS * 3];
+  if (ts.record)
+    for (int q = 0; q < ts.n * 3; q++) frec[q] = 0.f;
+  if (i < n_init) {
+    const uint32_t fl = st.flags[i];
+    if (i < N) {
+      // reset contact forces (src/leap_frog.cu:102-119)
+      V3 fc = v3(0., 0., 0.), ft = v3(0., 0., 0.), nn = v3(0., 0., 0.);
+      const bool eligible = !(fl & FL_BLANKED) && !(fl & FL_TOOL);
+      V3 vel = v3(st.vx[i], st.vy[i], st.vz[i]);
+      double T = st.T[i];
+      bool T_dirty = false;
+      if (eligible) {
+        const V3 qp = v3(st.x[i], st.y[i], st.z[i]);
+        for (int k = 0; k < ts.n; k++) {
+          const ToolView &tv = ts.t[k];
+          if (!tv.active) continue;
+          bool in = qp.x > tv.bbmin[0] && qp.x < tv.bbmax[0] && qp.y > tv.bbmin[1] &&
+                    qp.y < tv.bbmax[1] && qp.z > tv.bbmin[2] && qp.z < tv.bbmax[2];
+          if (!in) continue;
+          V3 cp = v3(0, 0, 0), nrm = v3(0, 0, 0);
+          if (tv.algo == MFG_CONTACT_HASHED) {
+            int ix = (int)(unsigned)((qp.x - tv.bbmin[0]) / tv.dx);
+            int iy = (int)(unsigned)((qp.y - tv.bbmin[1]) / tv.dx);
+            int iz = (int)(unsigned)((qp.z - tv.bbmin[2]) / tv.dx);
+            // inside the (moving) box the cell index is < n; clamp against rounding at the edge
+            ix = min(ix, tv.nx - 1);
+            iy = min(iy, tv.ny - 1);
+            iz = min(iz, tv.nz - 1);
+            if (!contact_by_ray(tv.tris, tv.cells, tv.ny, tv.nz, ix, iy, iz, qp, tv.bbmin, tv.dx))
+              continue;
+            contact_by_normal(tv.tris, tv.cells, tv.nx, tv.ny, tv.nz, ix, iy, iz, qp, cp, nrm);
+          } else {
+            double min_dist = FLT_MAX;
+            for (int q = 0; q < tv.n_tris; q++) {
+              V3 cc, n2;
+              closest_point(tv.tris[q], qp, cc, n2);
+              double d = dist2(cc, qp);
+              if (d < min_dist) {
+                min_dist = d;
+                cp = cc;
+                nrm = n2;
+              }
+            }
+            if (vdot(vsub(cp, qp), nrm) > 0.) continue;
+          }
+          const double dt = c.dt, dt2 = dt * dt;
+          double gN = vdot(vsub(qp, cp), nrm);
+          double s = -tv.mass * gN;
+          V3 fN = v3(s * nrm.x / dt2 * tv.alpha, s * nrm.y / dt2 * tv.alpha, s * nrm.z / dt2 * tv.alpha);
+          V3 fT = v3(0., 0., 0.);
+          if (tv.mu != 0.) {
+            V3 vv = vsub(vel, v3(tv.vel_const[0], tv.vel_const[1], tv.vel_const[2]));
+            V3 vr = v3(vv.x - vv.x * nrm.x, vv.y - vv.y * nrm.y, vv.z - vv.z * nrm.z);
+            double am = tv.alpha * tv.mass;
+            V3 kd = v3(am * vr.x / dt, am * vr.y / dt, am * vr.z / dt);
+            double fy = tv.mu * vlen(fN);
+            V3 fstar = vsub(ft, kd);
+            double lf = vlen(fstar);
+            if (lf > fy) fT = v3(fy * fstar.x / lf, fy * fstar.y / lf, fy * fstar.z / lf);
+            else fT = fstar;
+          }
+          fc = fN;
+          ft = fT;
+          nn = nrm;
+          if (ts.record) {
+            float a = (float)(fN.x + fT.x), b = (float)(fN.y + fT.y), d2 = (float)(fN.z + fT.z);
+            if (tv.algo != MFG_CONTACT_HASHED) {
+              a = isnan(a) ? 0.f : a;
+              b = isnan(b) ? 0.f : b;
+              d2 = isnan(d2) ? 0.f : d2;
+            }
+            frec[3 * k] += a;
+            frec[3 * k + 1] += b;
+            frec[3 * k + 2] += d2;
+          }
+        }
+        // friction heat (src/actions_gpu.cu:192-220)
+        if (c.a_fric) {
+          double fmag = sqrt_site(ft.x * ft.x + ft.y * ft.y + ft.z * ft.z, c.single_sqrt);
+          if (fmag != 0.) {
+            V3 vd = v3(vel.x - ts.fric_vel[0], vel.y - ts.fric_vel[1], vel.z - ts.fric_vel[2]);
+            double vdn = vd.x * nn.x + vd.y * nn.y + vd.z * nn.z;
+            V3 vr = v3(vd.x - vdn, vd.y - vdn, vd.z - vdn);
+            double vmag = sqrt_site(vr.x * vr.x + vr.y * vr.y + vr.z * vr.z, c.single_sqrt);
+            T += c.twp_eta * c.dt * fmag * vmag / (c.twp_cp * c.a_mass);
+            T_dirty = true;
+          }
+        }
+        // velocity boundary condition (src/actions_gpu.cu:464-487)
+        if (fl & FL_FIXED) {
+          st.vx[i] = st.vbx[i];
+          st.vy[i] = st.vby[i];
+          st.vz[i] = st.vbz[i];
+          fc = v3(0., 0., 0.);
+          st.xt[i] = 0.; st.yt[i] = 0.; st.zt[i] = 0.;
+          st.vxt[i] = 0.; st.vyt[i] = 0.; st.vzt[i] = 0.;
+        }
+      }
+      // thermal boundary condition (src/actions_gpu.cu:451-462)
+      if (c.a_thermal && (fl & FL_FIXED_ONE)) {
+        T = c.twp_T_init;
+        T_dirty = true;
+      }
+      if (T_dirty) st.T[i] = T;
+      st.fcx[i] = fc.x; st.fcy[i] = fc.y; st.fcz[i] = fc.z;
+      st.ftx[i] = ft.x; st.fty[i] = ft.y; st.ftz[i] = ft.z;
+    }
+    // tool particles move with the first tool (src/actions_gpu.cu:55-68)
+    if (fl & FL_TOOL) {
+      st.x[i] = st.x[i] + ts.fric_vel[0] * c.dt;
+      st.y[i] = st.y[i] + ts.fric_vel[1] * c.dt;
+      st.z[i] = st.z[i] + ts.fric_vel[2] * c.dt;
+    }
+  }
+  if (ts.record) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int q = 0; q < ts.n * 3; q++) {
+      float v = frec[q];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+      if (lane == 0) red[q][wid] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int q = 0; q < ts.n * 3; q++) {
+        float v = 0.f;
+        for (int w = 0; w < BST / 64; w++) v += red[q][w];
+        partials[(size_t)blockIdx.x * MAX_TOOLS * 3 + q] = v;
+      }
+  }
+}
+
+__global__ void k_force_reduce(const float *partials, int nblocks, int n_tools, float *forces) {
+  int q = threadIdx.x;
+  if (q >= n_tools * 3) return;
+  float s = 0.f;
+  for (int b = 0; b < nblocks; b++) s += partials[(size_t)b * MAX_TOOLS * 3 + q];
+  forces[q] += s;
+}
+
+// ---- NaN scan (debug_check_valid_full) ---------------------------------------------------------
+__global__ void k_check_valid(State st, unsigned n, int *flag) {
+  unsigned i = blockIdx.x * BST + threadIdx.x;
+  if (i >= n) return;
+  if (st.flags[i] & FL_BLANKED) return;
+  auto bad = [](double v) { return isnan(v) || isinf(v); };
+  bool b = bad(st.x[i]) || bad(st.y[i]) || bad(st.z[i]) || bad(st.vx[i]) || bad(st.vy[i]) ||
+           bad(st.vz[i]) || bad(st.rho[i]) || bad(st.T[i]) || bad(st.eps_pl[i]) ||
+           bad(st.eps_pl_dot[i]);
+  for (int k = 0; k < 6; k++) b = b || bad(st.S[k][i]);
+  if (b) atomicOr(flag, 1);
+}
+
+// ---------------------------------------------------------------------------------------------
+static inline unsigned nbk(long n) { return (unsigned)((n + BST - 1) / BST); }
+
+int tool_bin_triangles(hipStream_t s, const Tri *tris, int n_tris, const double bbmin[3],
+                       double dx, int nx, int ny, int nz, int **cells_out, int **tri_idx_out,
+                       int *n_boxed_out) {
+  long nc = (long)nx * ny * nz;
+  int *count = nullptr, *fill = nullptr, *idx = nullptr;
+  if (hipMalloc(&count, sizeof(int) * (nc + 1)) != hipSuccess) return MFG_ERR_NOMEM;
+  if (hipMalloc(&fill, sizeof(int) * (nc + 1)) != hipSuccess) {
+    (void)hipFree(count);
+    return MFG_ERR_NOMEM;
+  }
+  (void)hipMemsetAsync(count, 0, sizeof(int) * (nc + 1), s);
+  (void)hipMemsetAsync(fill, 0, sizeof(int) * (nc + 1), s);
+  TriGrid g;
+  for (int k = 0; k < 3; k++) g.bbmin[k] = bbmin[k];
+  g.dx = dx;
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  hipLaunchKernelGGL(k_tri_bin, dim3(nbk(n_tris)), dim3(BST), 0, s, tris, n_tris, g, count, fill,
+                     (int *)nullptr);
+  hipLaunchKernelGGL(k_tri_scan, dim3(1), dim3(1024), 0, s, count, (int)nc);
+  int total = 0;
+  if (hipMemcpyAsync(&total, count + nc, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    (void)hipFree(count);
+    (void)hipFree(fill);
+    return MFG_ERR_HIP;
+  }
+  if (hipMalloc(&idx, sizeof(int) * (total > 0 ? total : 1)) != hipSuccess) {
+    (void)hipFree(count);
+    (void)hipFree(fill);
+    return MFG_ERR_NOMEM;
+  }
+  hipLaunchKernelGGL(k_tri_bin, dim3(nbk(n_tris)), dim3(BST), 0, s, tris, n_tris, g, count, fill,
+                     idx);
+  hipLaunchKernelGGL(k_tri_cell_sort, dim3(nbk(nc)), dim3(BST), 0, s, count, (int)nc, idx);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(fill);
+  *cells_out = count;
+  *tri_idx_out = idx;
+  *n_boxed_out = total;
+  return MFG_OK;
+}
+
+void launch_gather_tris(hipStream_t s, const Tri *src, const int *idx, int n, Tri *dst) {
+  if (n > 0) hipLaunchKernelGGL(k_gather_tris, dim3(nbk(n)), dim3(BST), 0, s, src, idx, n, dst);
+}
+
+void launch_orient_test(hipStream_t s, const Tri *boxed, const int *cells, int nx, int ny, int nz,
+                        const double bbmin[3], double dx, const double q[3], int *inside) {
+  hipLaunchKernelGGL(k_orient, dim3(1), dim3(64), 0, s, boxed, cells, nx, ny, nz, bbmin[0],
+                     bbmin[1], bbmin[2], dx, q[0], q[1], q[2], inside);
+}
+
+void launch_flip_normals(hipStream_t s, Tri *tris, int n) {
+  if (n > 0) hipLaunchKernelGGL(k_flip, dim3(nbk(n)), dim3(BST), 0, s, tris, n);
+}
+
+void launch_tool_update(hipStream_t s, Tri *tris, const double *p_init, int n, double t,
+                        const double vel[3]) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_tool_update, dim3(nbk(n)), dim3(BST), 0, s, tris, p_init, n, t, vel[0],
+                       vel[1], vel[2]);
+}
+
+void launch_contact_bc(hipStream_t s, const State &st, unsigned n_init, const ToolSet &ts,
+                       const Consts &c, const Counters *cnt, float *partials, int nblocks) {
+  hipLaunchKernelGGL(k_contact_bc, dim3(nblocks), dim3(BST), 0, s, st, n_init, ts, c, cnt,
+                     partials);
+}
+
+void launch_force_reduce(hipStream_t s, const float *partials, int nblocks, int n_tools,
+                         float *forces) {
+  hipLaunchKernelGGL(k_force_reduce, dim3(1), dim3(64), 0, s, partials, nblocks, n_tools, forces);
+}
+
+int valid_check(hipStream_t s, const State &st, unsigned n, int *d_flag) {
+  (void)hipMemsetAsync(d_flag, 0, sizeof(int), s);
+  hipLaunchKernelGGL(k_check_valid, dim3(nbk(n)), dim3(BST), 0, s, st, n, d_flag);
+  int h = 0;
+  if (hipMemcpyAsync(&h, d_flag, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return MFG_ERR_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess) return MFG_ERR_HIP;
+  return h ? MFG_ERR_NONFINITE : MFG_OK;
+}
+
+}  // namespace mfg

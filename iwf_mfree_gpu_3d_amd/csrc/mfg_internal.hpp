@@ -1,0 +1,197 @@
+// mfg_internal.hpp -- device data layout and kernel launchers of the gfx950 SPH cutting step.
+//
+// Layout in HBM (one particle set):
+//   persistent state, double buffered and permuted every step (SoA, FP64):
+//     x y z | vx vy vz | vbx vby vbz | fcx fcy fcz ftx fty ftz | h rho | S[6] | eps_pl eps_pl_dot T
+//     | xt yt zt | vxt vyt vzt | rho_t T_t | St[6] | flags (u32) | id (u32)
+//   symmetric tensors are stored as xx, yy, zz, xy, yz, zx.
+//   half-step records (sorted order, rebuilt every step by the permute/predict kernel):
+//     nl4   : double4 (x, y, z, blanked)          -- candidate filter of the neighbour-list pass
+//     pse2  : double2 (T, rho)                    -- PSE heat pass (only with PSE)
+//     rec   : pair record, 192 B (FP64) / 112 B (FP32) -- everything a neighbour contributes
+//   neighbour list: int32 nbr[k * nl_stride + i], count[i]
+//   grid: key[N_init] (unsorted), key_sorted[N_init], order[N_init], cell counts / offsets
+//         [num_cell + 1], fill counters [num_cell]
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/mfree_gpu.h"
+
+namespace mfg {
+
+// particle flag bits (the reference stores these as floating point values and compares them
+// with == 1., == 0. or as truthy values; the bits keep each of those predicates)
+constexpr uint32_t FL_FIXED = 1u;      // fixed != 0        (src/actions_gpu.cu:470)
+constexpr uint32_t FL_BLANKED = 2u;    // blanked == 1      (everywhere)
+constexpr uint32_t FL_TOOL = 4u;       // tool_particle == 1 (everywhere)
+constexpr uint32_t FL_FIXED_ONE = 8u;  // fixed == 1        (src/actions_gpu.cu:455)
+constexpr uint32_t FL_TOOL_NZ = 16u;   // tool_particle != 0 (src/interactions_gpu.cu:167,230)
+
+constexpr int KEY_BLANK = 0x7fffffff;  // INT_MAX, hash of a hard-blanked particle
+
+struct State {
+  double *x, *y, *z;
+  double *vx, *vy, *vz;
+  double *vbx, *vby, *vbz;
+  double *fcx, *fcy, *fcz, *ftx, *fty, *ftz;
+  double *h, *rho;
+  double *S[6];
+  double *eps_pl, *eps_pl_dot, *T;
+  double *xt, *yt, *zt;
+  double *vxt, *vyt, *vzt;
+  double *rho_t, *T_t;
+  double *St[6];
+  uint32_t *flags;
+  uint32_t *id;
+};
+
+// number of double arrays in State (for allocation)
+constexpr int STATE_DOUBLES = 3 + 3 + 3 + 6 + 2 + 6 + 3 + 3 + 3 + 2 + 6;
+
+// pair record, FP64: 24 doubles
+struct alignas(16) Rec64 {
+  double x, y, z, h;
+  double vx, vy, vz, rho;
+  double A[6];  // (S - p I) / rho^2 for the diagonal, S / rho^2 off the diagonal
+  double R[6];  // artificial stress
+  double T, vol, c, tool;
+};
+
+// pair record, FP32 pair arithmetic: positions stay FP64
+struct alignas(16) Rec32 {
+  double x, y, z;
+  float h, vx, vy, vz, rho;
+  float A[6];
+  float R[6];
+  float T, vol, c, tool;
+  float pad;
+};
+
+static_assert(sizeof(Rec64) == 192, "Rec64 layout");
+static_assert(sizeof(Rec32) == 112, "Rec32 layout");
+
+struct Consts {
+  // actions translation unit (src/actions_gpu.cu:30-34)
+  double a_rho0, a_K, a_G, a_mass, a_stresseps;
+  double jc_A, jc_B, jc_C, jc_n, jc_m, jc_Tmelt, jc_Tref, jc_eps_dot_ref, jc_clamp;
+  double twp_cp, twp_tq, twp_T_init, twp_eta;
+  int plastic, a_thermal, a_fric, pad0;
+  // interactions translation unit (src/interactions_gpu.cu:26-34)
+  double i_mass, i_K, wdeltap, v_alpha, v_beta, v_eta, xspheps;
+  double alpha_wp, alpha_tool;
+  int pse_on, pad1;
+  // switches
+  int cspm, heat, single_sqrt, precision;
+  double dt;
+};
+
+struct GridInfo {
+  int nx, ny, nz;
+  int hard_blank;
+  double bbmin[3];
+  double dx;
+  long num_cell;
+};
+
+// device counters of one particle set
+struct Counters {
+  unsigned N;            // particles in cells (N of the reference)
+  unsigned n_blank_key;  // INT_MAX keys
+  int err_capacity;      // neighbour list overflow
+  int err_grid;          // unblanked particle outside the grid
+  int max_count;         // largest neighbour count of the last list build
+  int pad;
+  unsigned long long candidates;  // candidate pairs visited by the last neighbour pass
+  unsigned long long pairs;       // interacting pairs of the last neighbour pass
+};
+
+// triangle as stored on the device: 30 doubles (p1 p2 p3 n np1 np2 np3 ne1 ne2 ne3)
+struct Tri {
+  double p[3][3];
+  double n[3];
+  double np[3][3];
+  double ne[3][3];
+};
+
+// per-tool view passed to the contact kernel
+struct ToolView {
+  const Tri *tris;       // exhaustive: all triangles; hashing: boxed triangles
+  const int *cells;      // CSR offsets of the triangle grid (num_cell + 1)
+  int n_tris;            // triangles in `tris`
+  int algo;              // MFG_CONTACT_*
+  int nx, ny, nz;
+  int active;            // boxed triangles exist and birth <= t <= death
+  double bbmin[3], bbmax[3];  // moving bounding box (geometry constant of the reference)
+  double dx;
+  double mu, alpha, mass;
+  double vel_const[3];   // tool_vel constant (only set by set_vel)
+};
+
+constexpr int MAX_TOOLS = 8;
+struct ToolSet {
+  ToolView t[MAX_TOOLS];
+  int n;
+  int record;
+  double fric_vel[3];    // global_tool[0]->get_vel() for friction heat and tool particle motion
+};
+
+// ---------------------------------------------------------------------------------------------
+// kernel launchers (implemented in k_grid.hip, k_pair.hip, k_tool.hip)
+// ---------------------------------------------------------------------------------------------
+struct GridBuffers {
+  int *key;          // N_init, unsorted
+  int *key_sorted;   // N_init
+  int *order;        // N_init
+  int *count;        // num_cell + 1 (counts, then exclusive offsets)
+  int *fill;         // num_cell
+  int *blk;          // per-block blanked counts / offsets
+  int *blk2;         // scan scratch
+  int nblk;          // blocks of the hash kernel
+};
+
+void launch_blank_hash_count(hipStream_t s, const State &st, unsigned n_init, const GridInfo &g,
+                             int blanking, const double *bb, double vmax2, GridBuffers &gb,
+                             Counters *cnt);
+void launch_scan_cells(hipStream_t s, const GridInfo &g, GridBuffers &gb, unsigned n_init,
+                       Counters *cnt);
+void launch_scatter(hipStream_t s, unsigned n_init, const GridInfo &g, GridBuffers &gb,
+                    Counters *cnt);
+void launch_fixup(hipStream_t s, const GridInfo &g, GridBuffers &gb);
+void launch_permute_predict(hipStream_t s, const State &src, const State &dst, unsigned n_init,
+                            const GridBuffers &gb, const Counters *cnt, const Consts &c,
+                            int predict, void *rec, double4 *nl4, double2 *pse2);
+void launch_cells_to_tables(hipStream_t s, const GridInfo &g, const int *offsets, int rothlin,
+                            int *start, int *end);
+void launch_bbox_reduce(hipStream_t s, const State &st, const Counters *cnt, unsigned n_init,
+                        double *partials, int nblocks);
+
+void launch_neighbours(hipStream_t s, unsigned n_init, const GridInfo &g, const int *offsets,
+                       const int *key_sorted, const double4 *nl4, const double2 *pse2,
+                       const double *h, const uint32_t *flags, const Consts &c, int *nbr,
+                       int nl_stride, int cap, int *count, double *T_t, Counters *cnt,
+                       int want_pse);
+void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *rec, const int *nbr,
+                 int nl_stride, const int *count, const Consts &c, const Counters *cnt);
+
+void launch_tool_update(hipStream_t s, Tri *tris, const double *p_init, int n, double t,
+                        const double vel[3]);
+void launch_contact_bc(hipStream_t s, const State &st, unsigned n_init, const ToolSet &ts,
+                       const Consts &c, const Counters *cnt, float *partials, int nblocks);
+void launch_force_reduce(hipStream_t s, const float *partials, int nblocks, int n_tools,
+                         float *forces);
+
+// triangle binning on the GPU (grid_cpu_tri::get_cells)
+int tool_bin_triangles(hipStream_t s, const Tri *tris, int n_tris, const double bbmin[3],
+                       double dx, int nx, int ny, int nz, int **cells_out, int **tri_idx_out,
+                       int *n_boxed_out);
+void launch_gather_tris(hipStream_t s, const Tri *src, const int *idx, int n, Tri *dst);
+void launch_orient_test(hipStream_t s, const Tri *boxed, const int *cells, int nx, int ny,
+                        int nz, const double bbmin[3], double dx, const double q[3], int *inside);
+void launch_flip_normals(hipStream_t s, Tri *tris, int n);
+
+int valid_check(hipStream_t s, const State &st, unsigned n, int *d_flag);
+
+}  // namespace mfg
