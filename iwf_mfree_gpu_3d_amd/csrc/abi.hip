@@ -217,6 +217,7 @@ struct mfg_tool {
   Tri *boxed = nullptr;      // per-cell copies
   double *p_init_all = nullptr, *p_init_boxed = nullptr;
   int *cells = nullptr;      // CSR offsets, num_cell + 1
+  int *boxed_idx = nullptr;  // original triangle index of every boxed copy
   int gnx = 0, gny = 0, gnz = 0;
   double gbbmin[3], gbbmax[3], gdx = 0.;
   double bbmin[3], bbmax[3];  // moving copy (geometry constant of the reference)
@@ -1159,7 +1160,7 @@ int mfg_tool_3d_gpu_create(const double *verts3, int n_verts, const int *tris3, 
   t->gnz = (int)std::ceil((t->gbbmax[2] - t->gbbmin[2]) / t->gdx);
   hipStream_t s = g_lib.stream;
   auto bail = [&](int code) {
-    void *ptrs[] = {t->tris, t->boxed, t->p_init_all, t->p_init_boxed, t->cells};
+    void *ptrs[] = {t->tris, t->boxed, t->p_init_all, t->p_init_boxed, t->cells, t->boxed_idx};
     for (void *q : ptrs)
       if (q) (void)hipFree(q);
     delete t;
@@ -1172,10 +1173,9 @@ int mfg_tool_3d_gpu_create(const double *verts3, int n_verts, const int *tris3, 
   rc = tool_bin_triangles(s, t->tris, n_tris, t->gbbmin, t->gdx, t->gnx, t->gny, t->gnz, &t->cells,
                           &idx, &t->n_boxed);
   if (rc != MFG_OK) return bail(fail(rc, "triangle binning failed"));
-  if (hipMalloc(&t->boxed, sizeof(Tri) * std::max(t->n_boxed, 1)) != hipSuccess) {
-    (void)hipFree(idx);
+  t->boxed_idx = idx;
+  if (hipMalloc(&t->boxed, sizeof(Tri) * std::max(t->n_boxed, 1)) != hipSuccess)
     return bail(fail(MFG_ERR_NOMEM, "boxed triangles"));
-  }
   launch_gather_tris(s, t->tris, idx, t->n_boxed, t->boxed);
   // orient the normals so that a point just outside the grid box tests outside
   int *d_in = nullptr;
@@ -1192,7 +1192,6 @@ int mfg_tool_3d_gpu_create(const double *verts3, int n_verts, const int *tris3, 
     launch_flip_normals(s, t->boxed, t->n_boxed);
   }
   (void)hipFree(d_in);
-  (void)hipFree(idx);
   // initial vertex positions of both triangle sets (update_tool moves from these)
   std::vector<Tri> hb(std::max(t->n_boxed, 1));
   HIP_TRY(hipMemcpyAsync(hb.data(), t->boxed, sizeof(Tri) * t->n_boxed, hipMemcpyDeviceToHost, s));
@@ -1221,7 +1220,7 @@ int mfg_tool_3d_gpu_create(const double *verts3, int n_verts, const int *tris3, 
 int mfg_tool_destroy(mfg_tool *t) {
   if (!t) return MFG_OK;
   if (g_lib.ready) (void)hipStreamSynchronize(g_lib.stream);
-  void *ptrs[] = {t->tris, t->boxed, t->p_init_all, t->p_init_boxed, t->cells};
+  void *ptrs[] = {t->tris, t->boxed, t->p_init_all, t->p_init_boxed, t->cells, t->boxed_idx};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   delete t;
@@ -1324,7 +1323,11 @@ int mfg_tool_update(mfg_tool *t, double tc, double dt) {
   if (t->n_boxed == 0) return MFG_OK;
   ProfScope ps("tool_update");
   if (t->algo == MFG_CONTACT_HASHED) launch_tool_update(g_lib.stream, t->boxed, t->p_init_boxed, t->n_boxed, tc, t->vel);
-  else launch_tool_update(g_lib.stream, t->tris, t->p_init_all, t->n_tris, tc, t->vel);
+  else {
+    // the exhaustive search runs over the boxed copies too (grid-accelerated, same result)
+    launch_tool_update(g_lib.stream, t->tris, t->p_init_all, t->n_tris, tc, t->vel);
+    launch_tool_update(g_lib.stream, t->boxed, t->p_init_boxed, t->n_boxed, tc, t->vel);
+  }
   for (int k = 0; k < 3; k++) {
     t->bbmin[k] += dt * t->vel[k];
     t->bbmax[k] += dt * t->vel[k];
@@ -1445,6 +1448,8 @@ int mfg_leap_frog_step(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int rec
     v.tris = t->algo == MFG_CONTACT_HASHED ? t->boxed : t->tris;
     v.n_tris = t->algo == MFG_CONTACT_HASHED ? t->n_boxed : t->n_tris;
     v.cells = t->cells;
+    v.boxed = t->n_boxed > 0 ? t->boxed : nullptr;
+    v.boxed_idx = t->boxed_idx;
     v.nx = t->gnx; v.ny = t->gny; v.nz = t->gnz;
     v.active = (t->n_boxed > 0 && t_current >= t->birth && t_current <= t->death) ? 1 : 0;
     for (int a = 0; a < 3; a++) {

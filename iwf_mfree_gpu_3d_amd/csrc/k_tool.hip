@@ -242,6 +242,53 @@ __device__ bool contact_by_ray(const Tri *tris, const int *cells, int ny, int nz
   return (n_inter % 2) != 0;
 }
 
+// Closest triangle over ALL triangles of the mesh -- the result of the exhaustive loop of
+// do_compute_contact_forces_n2 (src/tool_3d_gpu.cu:501-513: smallest squared distance, first
+// index on ties) -- found by visiting the triangle grid in shells of growing Chebyshev distance
+// d around the particle's cell.  Every triangle not yet visited lies only in cells at distance
+// >= d, hence at least (d-1) dx away, so the search stops once the best squared distance is
+// below that bound (with a relative margin of 1e-9 against rounding of the moving grid).
+__device__ void closest_all_by_grid(const ToolView &tv, V3 qp, int ix, int iy, int iz, V3 &cp,
+                                    V3 &nrm) {
+  double best = FLT_MAX;
+  int best_i = 0x7fffffff;
+  const int nx = tv.nx, ny = tv.ny, nz = tv.nz;
+  const int dmax = max(max(max(ix, nx - 1 - ix), max(iy, ny - 1 - iy)), max(iz, nz - 1 - iz));
+  auto visit = [&](int a, int b, int cc) {
+    int bi = a * ny * nz + b * nz + cc;
+    for (int p = tv.cells[bi]; p < tv.cells[bi + 1]; p++) {
+      V3 c2, n2;
+      closest_point(tv.boxed[p], qp, c2, n2);
+      double d2 = dist2(c2, qp);
+      int oi = tv.boxed_idx[p];
+      if (d2 < best || (d2 == best && oi < best_i)) {
+        best = d2;
+        best_i = oi;
+        cp = c2;
+        nrm = n2;
+      }
+    }
+  };
+  for (int d = 0; d <= dmax; d++) {
+    if (d >= 1) {
+      double lb = (d - 1) * tv.dx * (1.0 - 1e-9);
+      if (best < lb * lb) break;
+    }
+    const int lo_i = max(ix - d, 0), hi_i = min(ix + d, nx - 1);
+    const int lo_j = max(iy - d, 0), hi_j = min(iy + d, ny - 1);
+    const int lo_k = max(iz - d, 0), hi_k = min(iz + d, nz - 1);
+    for (int a = lo_i; a <= hi_i; a++)
+      for (int b = lo_j; b <= hi_j; b++) {
+        if (a == ix - d || a == ix + d || b == iy - d || b == iy + d) {
+          for (int cc = lo_k; cc <= hi_k; cc++) visit(a, b, cc);
+        } else {
+          if (iz - d >= 0) visit(a, b, iz - d);
+          if (d > 0 && iz + d < nz) visit(a, b, iz + d);
+        }
+      }
+  }
+}
+
 // normal orientation test of the tool_3d constructor (host unhash: unsigned casts)
 __global__ void k_orient(const Tri *boxed, const int *cells, int nx, int ny, int nz, double b0,
                          double b1, double b2, double dx, double q0, double q1, double q2,
@@ -324,15 +371,22 @@ __global__ void __launch_bounds__(BST)
               continue;
             contact_by_normal(tv.tris, tv.cells, tv.nx, tv.ny, tv.nz, ix, iy, iz, qp, cp, nrm);
           } else {
-            double min_dist = FLT_MAX;
-            for (int q = 0; q < tv.n_tris; q++) {
-              V3 cc, n2;
-              closest_point(tv.tris[q], qp, cc, n2);
-              double d = dist2(cc, qp);
-              if (d < min_dist) {
-                min_dist = d;
-                cp = cc;
-                nrm = n2;
+            if (tv.boxed) {
+              int ix = min((int)(unsigned)((qp.x - tv.bbmin[0]) / tv.dx), tv.nx - 1);
+              int iy = min((int)(unsigned)((qp.y - tv.bbmin[1]) / tv.dx), tv.ny - 1);
+              int iz = min((int)(unsigned)((qp.z - tv.bbmin[2]) / tv.dx), tv.nz - 1);
+              closest_all_by_grid(tv, qp, ix, iy, iz, cp, nrm);
+            } else {
+              double min_dist = FLT_MAX;
+              for (int q = 0; q < tv.n_tris; q++) {
+                V3 cc, n2;
+                closest_point(tv.tris[q], qp, cc, n2);
+                double d = dist2(cc, qp);
+                if (d < min_dist) {
+                  min_dist = d;
+                  cp = cc;
+                  nrm = n2;
+                }
               }
             }
             if (vdot(vsub(cp, qp), nrm) > 0.) continue;

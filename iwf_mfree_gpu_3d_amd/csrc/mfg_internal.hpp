@@ -119,6 +119,8 @@ struct Tri {
 // per-tool view passed to the contact kernel
 struct ToolView {
   const Tri *tris;       // exhaustive: all triangles; hashing: boxed triangles
+  const Tri *boxed;      // per-cell triangle copies (both algorithms)
+  const int *boxed_idx;  // original index of every boxed triangle
   const int *cells;      // CSR offsets of the triangle grid (num_cell + 1)
   int n_tris;            // triangles in `tris`
   int algo;              // MFG_CONTACT_*
