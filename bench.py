@@ -27,10 +27,12 @@ HBM_PEAK_GBS = 8000.0
 FP64_VECTOR_PEAK_TFLOPS = 78.6
 FP32_VECTOR_PEAK_TFLOPS = 157.3
 
-# algorithmic work per unit (SURVEY-derived figures restated in DESIGN.md):
+# algorithmic work per unit:
 #   pair pass: 280 FP operations per interacting pair (CSPM + B 55, grad W 25, grad v 30,
 #              artificial stress 15, stress divergence 63, viscosity 40, XSPH 10, Brookshaw 25)
 #   neighbour pass: 11 per candidate (r^2 and support test) + 10 per candidate for the PSE sum
+#   algorithmic HBM bytes of the pair pass per particle: its own record and state read once,
+#   its neighbour list (4 B per pair), the updated state written once (FP64 field counts below)
 FLOPS_PER_PAIR = 280.0
 FLOPS_PER_CANDIDATE = 11.0
 FLOPS_PER_PSE_CANDIDATE = 10.0

@@ -218,6 +218,10 @@ struct mfg_tool {
   double *p_init_all = nullptr, *p_init_boxed = nullptr;
   int *cells = nullptr;      // CSR offsets, num_cell + 1
   int *boxed_idx = nullptr;  // original triangle index of every boxed copy
+  int *cand_off = nullptr;   // closest-triangle candidate lists (exhaustive algorithm only)
+  int *cand_idx = nullptr;
+  long n_cand = 0;
+  double mesh_tc = 0.;       // time of the last translation of the triangles
   int gnx = 0, gny = 0, gnz = 0;
   double gbbmin[3], gbbmax[3], gdx = 0.;
   double bbmin[3], bbmax[3];  // moving copy (geometry constant of the reference)
@@ -1160,7 +1164,8 @@ int mfg_tool_3d_gpu_create(const double *verts3, int n_verts, const int *tris3, 
   t->gnz = (int)std::ceil((t->gbbmax[2] - t->gbbmin[2]) / t->gdx);
   hipStream_t s = g_lib.stream;
   auto bail = [&](int code) {
-    void *ptrs[] = {t->tris, t->boxed, t->p_init_all, t->p_init_boxed, t->cells, t->boxed_idx};
+    void *ptrs[] = {t->tris, t->boxed, t->p_init_all, t->p_init_boxed, t->cells, t->boxed_idx,
+                    t->cand_off, t->cand_idx};
     for (void *q : ptrs)
       if (q) (void)hipFree(q);
     delete t;
@@ -1220,7 +1225,8 @@ int mfg_tool_3d_gpu_create(const double *verts3, int n_verts, const int *tris3, 
 int mfg_tool_destroy(mfg_tool *t) {
   if (!t) return MFG_OK;
   if (g_lib.ready) (void)hipStreamSynchronize(g_lib.stream);
-  void *ptrs[] = {t->tris, t->boxed, t->p_init_all, t->p_init_boxed, t->cells, t->boxed_idx};
+  void *ptrs[] = {t->tris, t->boxed, t->p_init_all, t->p_init_boxed, t->cells, t->boxed_idx,
+                  t->cand_off, t->cand_idx};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   delete t;
@@ -1245,6 +1251,19 @@ int mfg_tool_set_contact_alpha(mfg_tool *t, double a) {
 int mfg_tool_set_algorithm_type(mfg_tool *t, int algo) {
   if (!t || (algo != MFG_CONTACT_EXHAUSTIVE && algo != MFG_CONTACT_HASHED)) return fail(MFG_ERR_ARG, "algorithm %d", algo);
   t->algo = algo;
+  if (algo == MFG_CONTACT_EXHAUSTIVE && !t->cand_off) {
+    // candidate lists are built from the untranslated mesh, in the grid's initial frame
+    NEED_LIB();
+    hipStream_t s = g_lib.stream;
+    Tri *tmp = nullptr;
+    HIP_TRY(hipMalloc(&tmp, sizeof(Tri) * t->n_tris));
+    HIP_TRY(hipMemcpyAsync(tmp, t->tris, sizeof(Tri) * t->n_tris, hipMemcpyDeviceToDevice, s));
+    launch_tool_update(s, tmp, t->p_init_all, t->n_tris, 0., t->vel);
+    int rc = tool_candidate_lists(s, tmp, t->n_tris, t->gbbmin, t->gdx, t->gnx, t->gny, t->gnz,
+                                  &t->cand_off, &t->cand_idx, &t->n_cand);
+    (void)hipFree(tmp);
+    if (rc != MFG_OK) return fail(rc, "closest-triangle candidate lists");
+  }
   return MFG_OK;
 }
 int mfg_tool_set_birth_death(mfg_tool *t, double b, double d) {
@@ -1328,6 +1347,7 @@ int mfg_tool_update(mfg_tool *t, double tc, double dt) {
     launch_tool_update(g_lib.stream, t->tris, t->p_init_all, t->n_tris, tc, t->vel);
     launch_tool_update(g_lib.stream, t->boxed, t->p_init_boxed, t->n_boxed, tc, t->vel);
   }
+  t->mesh_tc = tc;
   for (int k = 0; k < 3; k++) {
     t->bbmin[k] += dt * t->vel[k];
     t->bbmax[k] += dt * t->vel[k];
@@ -1448,8 +1468,9 @@ int mfg_leap_frog_step(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int rec
     v.tris = t->algo == MFG_CONTACT_HASHED ? t->boxed : t->tris;
     v.n_tris = t->algo == MFG_CONTACT_HASHED ? t->n_boxed : t->n_tris;
     v.cells = t->cells;
-    v.boxed = t->n_boxed > 0 ? t->boxed : nullptr;
-    v.boxed_idx = t->boxed_idx;
+    v.cand_off = t->algo == MFG_CONTACT_EXHAUSTIVE ? t->cand_off : nullptr;
+    v.cand_idx = t->cand_idx;
+    for (int a = 0; a < 3; a++) v.corg[a] = t->gbbmin[a] + t->vel[a] * t->mesh_tc;
     v.nx = t->gnx; v.ny = t->gny; v.nz = t->gnz;
     v.active = (t->n_boxed > 0 && t_current >= t->birth && t_current <= t->death) ? 1 : 0;
     for (int a = 0; a < 3; a++) {

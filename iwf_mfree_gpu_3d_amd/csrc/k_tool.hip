@@ -242,50 +242,114 @@ __device__ bool contact_by_ray(const Tri *tris, const int *cells, int ny, int nz
   return (n_inter % 2) != 0;
 }
 
-// Closest triangle over ALL triangles of the mesh -- the result of the exhaustive loop of
-// do_compute_contact_forces_n2 (src/tool_3d_gpu.cu:501-513: smallest squared distance, first
-// index on ties) -- found by visiting the triangle grid in shells of growing Chebyshev distance
-// d around the particle's cell.  Every triangle not yet visited lies only in cells at distance
-// >= d, hence at least (d-1) dx away, so the search stops once the best squared distance is
-// below that bound (with a relative margin of 1e-9 against rounding of the moving grid).
-__device__ void closest_all_by_grid(const ToolView &tv, V3 qp, int ix, int iy, int iz, V3 &cp,
-                                    V3 &nrm) {
+// ---- closest-triangle candidate lists ----------------------------------------------------------
+// The exhaustive contact search of do_compute_contact_forces_n2 (src/tool_3d_gpu.cu:501-513) takes
+// the triangle of smallest squared distance over ALL triangles, the first index on ties.  Here
+// every cell of the triangle grid keeps the triangles that can be that argmin for a query point
+// in the cell, ascending by index, so scanning the list with a strict '<' returns the same
+// triangle as the full loop.  With the cell box widened by `slack` on every side, corners c_k,
+// centre m and half diagonal r:
+//   ub = min_T max_k |c_k - T| bounds the closest distance of every point of the box from above
+//        (the distance to a convex set is convex, so its maximum over a box is at a corner);
+//   |m - T| - r <= |p - T| for every point p of the box;
+// so a triangle with |m - T| - r > ub is never the closest one.  A relative margin of 1e-6 on
+// the cut covers rounding.  The grid moves rigidly with the mesh: the lists hold for all time.
+__device__ void cell_geometry(const TriGrid &g, int cell, double slack, V3 c[8], V3 &m,
+                              double &r) {
+  const int a = cell / (g.ny * g.nz), b = (cell / g.nz) % g.ny, cc = cell % g.nz;
+  const double lo[3] = {g.bbmin[0] + a * g.dx - slack, g.bbmin[1] + b * g.dx - slack,
+                        g.bbmin[2] + cc * g.dx - slack};
+  const double w = g.dx + 2. * slack;
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    c[k] = v3(lo[0] + ((k & 1) ? w : 0.), lo[1] + ((k & 2) ? w : 0.), lo[2] + ((k & 4) ? w : 0.));
+  m = v3(lo[0] + 0.5 * w, lo[1] + 0.5 * w, lo[2] + 0.5 * w);
+  r = 0.5 * w * sqrt_rn(3.);
+}
+
+__device__ __forceinline__ double tri_dist2(const Tri &t, V3 q) {
+  V3 cp, n;
+  closest_point(t, q, cp, n);
+  return dist2(cp, q);
+}
+
+static_assert(BST == 256, "the block reductions below assume four wavefronts");
+
+// block of 256 per cell: the cut radius^2 of the cell and the number of candidates
+__global__ void __launch_bounds__(BST)
+    k_cand_count(const Tri *tris, int n, TriGrid g, double slack, double *cut2, int *count) {
+  __shared__ double red[BST / 64];
+  __shared__ int redi[BST / 64];
+  const int cell = blockIdx.x;
+  V3 c[8], m;
+  double r;
+  cell_geometry(g, cell, slack, c, m, r);
+  double best = DBL_MAX;
+  for (int t = threadIdx.x; t < n; t += BST) {
+    double mx = 0.;
+#pragma unroll
+    for (int k = 0; k < 8; k++) mx = fmax(mx, tri_dist2(tris[t], c[k]));
+    best = fmin(best, mx);
+  }
+  for (int o = 32; o > 0; o >>= 1) best = fmin(best, __shfl_xor(best, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = best;
+  __syncthreads();
+  best = fmin(fmin(red[0], red[1]), fmin(red[2], red[3]));
+  const double ub = sqrt_rn(best);
+  const double lim = (ub + r) * (1. + 1e-6);
+  const double lim2 = lim * lim;
+  int cnt = 0;
+  for (int t = threadIdx.x; t < n; t += BST) cnt += tri_dist2(tris[t], m) <= lim2 ? 1 : 0;
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+  if ((threadIdx.x & 63) == 0) redi[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    count[cell] = redi[0] + redi[1] + redi[2] + redi[3];
+    cut2[cell] = lim2;
+  }
+}
+
+// block of 256 per cell: write the candidates in ascending index order
+__global__ void __launch_bounds__(BST)
+    k_cand_fill(const Tri *tris, int n, TriGrid g, double slack, const double *cut2,
+                const int *off, int *idx) {
+  __shared__ int wsum[BST / 64];
+  const int cell = blockIdx.x;
+  V3 c[8], m;
+  double r;
+  cell_geometry(g, cell, slack, c, m, r);
+  const double lim2 = cut2[cell];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int base = off[cell];
+  const int end = off[cell + 1];  // the count pass evaluated the same predicate
+  for (int t0 = 0; t0 < n; t0 += BST) {
+    const int t = t0 + threadIdx.x;
+    const bool take = t < n && tri_dist2(tris[t], m) <= lim2;
+    const unsigned long long mask = __ballot(take);
+    const int before = __popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[w] = __popcll(mask);
+    __syncthreads();
+    int wbase = 0;
+    for (int k = 0; k < w; k++) wbase += wsum[k];
+    if (take && base + wbase + before < end) idx[base + wbase + before] = t;
+    base += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+  }
+}
+
+// closest triangle of the whole mesh for a point in cell `cell` (see above)
+__device__ __forceinline__ void closest_by_candidates(const ToolView &tv, V3 qp, int cell, V3 &cp,
+                                                      V3 &nrm) {
   double best = FLT_MAX;
-  int best_i = 0x7fffffff;
-  const int nx = tv.nx, ny = tv.ny, nz = tv.nz;
-  const int dmax = max(max(max(ix, nx - 1 - ix), max(iy, ny - 1 - iy)), max(iz, nz - 1 - iz));
-  auto visit = [&](int a, int b, int cc) {
-    int bi = a * ny * nz + b * nz + cc;
-    for (int p = tv.cells[bi]; p < tv.cells[bi + 1]; p++) {
-      V3 c2, n2;
-      closest_point(tv.boxed[p], qp, c2, n2);
-      double d2 = dist2(c2, qp);
-      int oi = tv.boxed_idx[p];
-      if (d2 < best || (d2 == best && oi < best_i)) {
-        best = d2;
-        best_i = oi;
-        cp = c2;
-        nrm = n2;
-      }
+  for (int p = tv.cand_off[cell]; p < tv.cand_off[cell + 1]; p++) {
+    V3 c2, n2;
+    closest_point(tv.tris[tv.cand_idx[p]], qp, c2, n2);
+    const double d2 = dist2(c2, qp);
+    if (d2 < best) {
+      best = d2;
+      cp = c2;
+      nrm = n2;
     }
-  };
-  for (int d = 0; d <= dmax; d++) {
-    if (d >= 1) {
-      double lb = (d - 1) * tv.dx * (1.0 - 1e-9);
-      if (best < lb * lb) break;
-    }
-    const int lo_i = max(ix - d, 0), hi_i = min(ix + d, nx - 1);
-    const int lo_j = max(iy - d, 0), hi_j = min(iy + d, ny - 1);
-    const int lo_k = max(iz - d, 0), hi_k = min(iz + d, nz - 1);
-    for (int a = lo_i; a <= hi_i; a++)
-      for (int b = lo_j; b <= hi_j; b++) {
-        if (a == ix - d || a == ix + d || b == iy - d || b == iy + d) {
-          for (int cc = lo_k; cc <= hi_k; cc++) visit(a, b, cc);
-        } else {
-          if (iz - d >= 0) visit(a, b, iz - d);
-          if (d > 0 && iz + d < nz) visit(a, b, iz + d);
-        }
-      }
   }
 }
 
@@ -371,11 +435,14 @@ __global__ void __launch_bounds__(BST)
               continue;
             contact_by_normal(tv.tris, tv.cells, tv.nx, tv.ny, tv.nz, ix, iy, iz, qp, cp, nrm);
           } else {
-            if (tv.boxed) {
-              int ix = min((int)(unsigned)((qp.x - tv.bbmin[0]) / tv.dx), tv.nx - 1);
-              int iy = min((int)(unsigned)((qp.y - tv.bbmin[1]) / tv.dx), tv.ny - 1);
-              int iz = min((int)(unsigned)((qp.z - tv.bbmin[2]) / tv.dx), tv.nz - 1);
-              closest_all_by_grid(tv, qp, ix, iy, iz, cp, nrm);
+            // cell of the point in the mesh's own frame; outside the grid, the full loop
+            const double fx = (qp.x - tv.corg[0]) / tv.dx, fy = (qp.y - tv.corg[1]) / tv.dx,
+                         fz = (qp.z - tv.corg[2]) / tv.dx;
+            const bool in_grid = tv.cand_off && fx >= 0. && fy >= 0. && fz >= 0. &&
+                                 fx < tv.nx && fy < tv.ny && fz < tv.nz;
+            if (in_grid) {
+              const int cell = ((int)fx * tv.ny + (int)fy) * tv.nz + (int)fz;
+              closest_by_candidates(tv, qp, cell, cp, nrm);
             } else {
               double min_dist = FLT_MAX;
               for (int q = 0; q < tv.n_tris; q++) {
@@ -542,6 +609,55 @@ int tool_bin_triangles(hipStream_t s, const Tri *tris, int n_tris, const double 
   *cells_out = count;
   *tri_idx_out = idx;
   *n_boxed_out = total;
+  return MFG_OK;
+}
+
+int tool_candidate_lists(hipStream_t s, const Tri *tris, int n_tris, const double bbmin[3],
+                         double dx, int nx, int ny, int nz, int **off_out, int **idx_out,
+                         long *total_out) {
+  const long nc = (long)nx * ny * nz;
+  if (nc <= 0 || nc > 0x7fffffffL) return MFG_ERR_ARG;
+  TriGrid g;
+  for (int k = 0; k < 3; k++) g.bbmin[k] = bbmin[k];
+  g.dx = dx;
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  const double slack = 1e-3 * dx;
+  int *off = nullptr, *idx = nullptr;
+  double *cut2 = nullptr;
+  if (hipMalloc(&off, sizeof(int) * (nc + 1)) != hipSuccess) return MFG_ERR_NOMEM;
+  if (hipMalloc(&cut2, sizeof(double) * nc) != hipSuccess) {
+    (void)hipFree(off);
+    return MFG_ERR_NOMEM;
+  }
+  hipLaunchKernelGGL(k_cand_count, dim3((unsigned)nc), dim3(BST), 0, s, tris, n_tris, g, slack,
+                     cut2, off);
+  hipLaunchKernelGGL(k_tri_scan, dim3(1), dim3(1024), 0, s, off, (int)nc);
+  int total = 0;
+  if (hipMemcpyAsync(&total, off + nc, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess || total < 0) {
+    (void)hipFree(off);
+    (void)hipFree(cut2);
+    return MFG_ERR_HIP;
+  }
+  if (hipMalloc(&idx, sizeof(int) * (total > 0 ? total : 1)) != hipSuccess) {
+    (void)hipFree(off);
+    (void)hipFree(cut2);
+    return MFG_ERR_NOMEM;
+  }
+  hipLaunchKernelGGL(k_cand_fill, dim3((unsigned)nc), dim3(BST), 0, s, tris, n_tris, g, slack,
+                     cut2, off, idx);
+  const hipError_t e = hipStreamSynchronize(s);
+  (void)hipFree(cut2);
+  if (e != hipSuccess) {
+    (void)hipFree(off);
+    (void)hipFree(idx);
+    return MFG_ERR_HIP;
+  }
+  *off_out = off;
+  *idx_out = idx;
+  *total_out = total;
   return MFG_OK;
 }
 

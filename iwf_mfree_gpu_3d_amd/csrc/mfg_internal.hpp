@@ -119,9 +119,10 @@ struct Tri {
 // per-tool view passed to the contact kernel
 struct ToolView {
   const Tri *tris;       // exhaustive: all triangles; hashing: boxed triangles
-  const Tri *boxed;      // per-cell triangle copies (both algorithms)
-  const int *boxed_idx;  // original index of every boxed triangle
-  const int *cells;      // CSR offsets of the triangle grid (num_cell + 1)
+  const int *cells;      // CSR offsets of the triangle grid (num_cell + 1), hashing
+  const int *cand_off;   // exhaustive: closest-triangle candidate lists per grid cell (CSR)
+  const int *cand_idx;   //   (triangle indices, ascending within a cell)
+  double corg[3];        // exhaustive: grid origin in the mesh's current position
   int n_tris;            // triangles in `tris`
   int algo;              // MFG_CONTACT_*
   int nx, ny, nz;
@@ -189,6 +190,10 @@ void launch_force_reduce(hipStream_t s, const float *partials, int nblocks, int 
 int tool_bin_triangles(hipStream_t s, const Tri *tris, int n_tris, const double bbmin[3],
                        double dx, int nx, int ny, int nz, int **cells_out, int **tri_idx_out,
                        int *n_boxed_out);
+// closest-triangle candidate lists of every cell of the triangle grid (exhaustive contact)
+int tool_candidate_lists(hipStream_t s, const Tri *tris, int n_tris, const double bbmin[3],
+                         double dx, int nx, int ny, int nz, int **off_out, int **idx_out,
+                         long *total_out);
 void launch_gather_tris(hipStream_t s, const Tri *src, const int *idx, int n, Tri *dst);
 void launch_orient_test(hipStream_t s, const Tri *boxed, const int *cells, int nx, int ny,
                         int nz, const double bbmin[3], double dx, const double q[3], int *inside);
