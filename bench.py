@@ -50,7 +50,7 @@ def gpu_run(args):
     log(f"scene preset {args.preset} scale {args.scale}: {scene.n} particles "
         f"({scene.params.n_workpiece} workpiece), {sum(t[1].shape[0] for t in scene.tools)} tool triangles, "
         f"built in {time.time() - t0:.1f}s")
-    prec = _lib.MFG_FP64 if args.precision == "fp64" else _lib.MFG_FP32_KAHAN
+    prec = {"fp64": _lib.MFG_FP64, "fp32-kahan": _lib.MFG_FP32_KAHAN, "fp32-wide": _lib.MFG_FP32_WIDE}[args.precision]
     parts, grid = scenes.setup_cut(scene, precision=prec, single_sqrt=True)
     lf = api.leap_frog(scene.n, grid.num_cell())
     api.global_time_current = api.global_time_dt
@@ -117,6 +117,19 @@ def cpu_baseline(args):
                       f"{secs:.1f} s"}
 
 
+def copy_probe(mib: int) -> None:
+    """A plain device copy of known size (y = 2 x over `mib` MiB of float32), used to scale the
+    FETCH_SIZE / WRITE_SIZE counters of the profiler to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.rand(n, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        torch.mul(a, 2.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,10 +138,15 @@ def main():
     ap.add_argument("--preset", type=int, default=2)
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "fp32-kahan", "fp32-wide"], default="fp64")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain copy of this many MiB (profiler counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the step shards by x slabs, see DESIGN.md)", file=sys.stderr)

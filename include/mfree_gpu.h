@@ -43,7 +43,9 @@ enum {
 };
 
 /* ---- enumerations ------------------------------------------------------------------------ */
-enum { MFG_FP64 = 0, MFG_FP32_KAHAN = 1 };                        /* precision of the pair pass */
+/* precision of the pair pass: FP64 throughout; FP32 pair arithmetic with Kahan-compensated FP32
+ * neighbour sums; FP32 pair arithmetic with the neighbour sums widened to FP64 */
+enum { MFG_FP64 = 0, MFG_FP32_KAHAN = 1, MFG_FP32_WIDE = 2 };
 enum { MFG_HEAT_NONE = 0, MFG_HEAT_BROOKSHAW = 1, MFG_HEAT_PSE = 2 }; /* src/types.h:104-105 */
 enum { MFG_CONTACT_EXHAUSTIVE = 0, MFG_CONTACT_HASHED = 1 };      /* src/tool_3d_gpu.h:77-79 */
 enum { MFG_GRID_GREEN = 0, MFG_GRID_ROTHLIN = 1 };

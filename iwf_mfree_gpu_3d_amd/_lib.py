@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libmfree_hip.so")
 MFG_OK = 0
 MFG_ERR_ARG, MFG_ERR_HIP, MFG_ERR_STATE, MFG_ERR_CAPACITY = -1, -2, -3, -4
 MFG_ERR_ALL_BLANKED, MFG_ERR_GRID, MFG_ERR_NOMEM, MFG_ERR_NONFINITE = -5, -6, -7, -8
-MFG_FP64, MFG_FP32_KAHAN = 0, 1
+MFG_FP64, MFG_FP32_KAHAN, MFG_FP32_WIDE = 0, 1, 2
 MFG_HEAT_NONE, MFG_HEAT_BROOKSHAW, MFG_HEAT_PSE = 0, 1, 2
 MFG_CONTACT_EXHAUSTIVE, MFG_CONTACT_HASHED = 0, 1
 MFG_GRID_GREEN, MFG_GRID_ROTHLIN = 0, 1

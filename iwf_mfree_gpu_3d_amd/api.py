@@ -18,7 +18,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (MFG_CONTACT_EXHAUSTIVE, MFG_CONTACT_HASHED, MFG_FP32_KAHAN, MFG_FP64, MFG_GRID_GREEN,
+from ._lib import (MFG_CONTACT_EXHAUSTIVE, MFG_CONTACT_HASHED, MFG_FP32_KAHAN, MFG_FP32_WIDE, MFG_FP64, MFG_GRID_GREEN,
                    MFG_GRID_ROTHLIN, MFG_HEAT_BROOKSHAW, MFG_HEAT_NONE, MFG_HEAT_PSE, MfgError, check,
                    corr_constants, joco_constants, phys_constants, trml_constants)
 

@@ -65,7 +65,26 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_driver(verbose)
     return LIB
+
+
+DRIVER_SRC = os.path.join(HERE, "driver", "mfree_gpu_3d.cpp")
+DRIVER = os.path.join(HERE, "mfree_gpu_3d")
+
+
+def build_driver(verbose: bool = False) -> str:
+    """The C++ host driver (reference-shaped main over the C++ shim), linked to the library with
+    an rpath next to it."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", DRIVER_SRC, "-o", DRIVER + ".tmp",
+           f"-L{HERE}", "-lmfree_hip", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    if verbose:
+        print(f"built {DRIVER}")
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -554,6 +554,8 @@ int mfg_set_options(const mfg_options *o) {
   if (o->heat < MFG_HEAT_NONE || o->heat > MFG_HEAT_PSE) return fail(MFG_ERR_ARG, "heat model %d", o->heat);
   g_lib.opt = *o;
   if (g_lib.opt.max_neighbours <= 0) g_lib.opt.max_neighbours = 256;
+  // list rows are read as 16-byte vectors: a multiple of 4 entries, at least 8
+  g_lib.opt.max_neighbours = std::max(8, (g_lib.opt.max_neighbours + 3) / 4 * 4);
   return MFG_OK;
 }
 
@@ -643,7 +645,8 @@ int mfg_particle_gpu_create(unsigned N, const double *pos4, const double *vel4, 
                             mfg_particles **out) {
   NEED_LIB();
   if (!out || !pos4 || N == 0) return fail(MFG_ERR_ARG, "particle_gpu: need N > 0 and positions");
-  if (precision != MFG_FP64 && precision != MFG_FP32_KAHAN) return fail(MFG_ERR_ARG, "precision %d", precision);
+  if (precision != MFG_FP64 && precision != MFG_FP32_KAHAN && precision != MFG_FP32_WIDE)
+    return fail(MFG_ERR_ARG, "precision %d", precision);
   mfg_particles *p = new mfg_particles();
   p->N_init = N;
   p->N_host = N;
@@ -662,7 +665,7 @@ int mfg_particle_gpu_create(unsigned N, const double *pos4, const double *vel4, 
   }
   size_t rec_bytes = precision == MFG_FP64 ? sizeof(Rec64) : sizeof(Rec32);
   p->cap = g_lib.opt.max_neighbours > 0 ? g_lib.opt.max_neighbours : 256;
-  p->nl_stride = (int)n;
+  p->nl_stride = p->cap;  // row-major lists: nbr[i * cap + k]
   p->nblk = (int)((N + 255) / 256);
   p->contact_blocks = p->nblk;
   if (hipMalloc(&p->rec, rec_bytes * n) != hipSuccess || hipMalloc(&p->nl4, sizeof(double4) * n) != hipSuccess ||
@@ -1121,11 +1124,11 @@ int mfg_grid_neighbours(mfg_grid *g, mfg_particles *p, int *h_counts, int *h_lis
   HIP_TRY(hipMemcpy(cnts.data(), p->count, sizeof(int) * N, hipMemcpyDeviceToHost));
   for (unsigned i = 0; i < N; i++) h_counts[i] = cnts[i];
   if (h_lists && cap > 0) {
-    std::vector<int> nb((size_t)p->cap * p->nl_stride);
+    std::vector<int> nb((size_t)p->nl_stride * N);
     HIP_TRY(hipMemcpy(nb.data(), p->nbr, sizeof(int) * nb.size(), hipMemcpyDeviceToHost));
     for (unsigned i = 0; i < N; i++)
       for (int k = 0; k < std::min(cnts[i], cap); k++)
-        h_lists[(size_t)i * cap + k] = nb[(size_t)k * p->nl_stride + i];
+        h_lists[(size_t)i * cap + k] = nb[(size_t)i * p->nl_stride + k];
   }
   return rc;
 }
@@ -1449,7 +1452,9 @@ int mfg_leap_frog_step(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int rec
                       p->st().T_t, p->cnt, pse ? 1 : 0);
   }
   {
-    ProfScope ps(p->precision == MFG_FP64 ? "pair_update_fp64" : "pair_update_fp32");
+    ProfScope ps(p->precision == MFG_FP64          ? "pair_update_fp64"
+                 : p->precision == MFG_FP32_KAHAN ? "pair_update_fp32_kahan"
+                                                  : "pair_update_fp32_wide");
     launch_pair(s, p->N_init, p->st(), p->rec, p->nbr, p->nl_stride, p->count, c, p->cnt);
   }
   // tools

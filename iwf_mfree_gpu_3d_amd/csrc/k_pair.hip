@@ -24,6 +24,17 @@
 namespace mfg {
 
 constexpr int BSP = 256;
+// minimum waves per SIMD the pair kernels are compiled for (register budget)
+#ifndef PAIR32_WAVES
+#define PAIR32_WAVES 2
+#endif
+#ifndef PAIR64_WAVES
+#define PAIR64_WAVES 1
+#endif
+// 1: the gather of neighbour k + 1 is issued before the arithmetic of neighbour k (FP64 pass)
+#ifndef PAIR64_PIPELINE
+#define PAIR64_PIPELINE 1
+#endif
 constexpr double PI_D = 3.14159265358979323846;
 constexpr double INV_PI_D = 0.31830988618379067154;
 
@@ -34,7 +45,7 @@ __global__ void __launch_bounds__(BSP)
                  Consts c, int *nbr, int stride, int cap, int *count, double *T_t, Counters *cnt,
                  int want_pse) {
   __shared__ unsigned long long red[2][BSP / 64];
-  const unsigned i = blockIdx.x * BSP + threadIdx.x;
+  const unsigned i = xcd_block() * BSP + threadIdx.x;
   const unsigned N = cnt->N;
   unsigned long long n_cand = 0, n_pair = 0;
   if (i < N && i < n_init) {
@@ -75,7 +86,7 @@ __global__ void __launch_bounds__(BSP)
           Tti += (tr.x - Ti) * w2 * c.i_mass / tr.y;
         }
         if (!blanked_i && pj.w == 0. && !(rr2 >= fourh2 || rr2 < 1e-8)) {
-          if (n < cap) nbr[(size_t)n * stride + i] = j;
+          if (n < cap) nbr[(size_t)i * stride + n] = j;
           n++;
         }
       }
@@ -110,6 +121,36 @@ __global__ void __launch_bounds__(BSP)
     }
     if (a) atomicAdd(&cnt->candidates, a);
     if (b) atomicAdd(&cnt->pairs, b);
+  }
+}
+
+// Runs body(load(row[k])) for k = 0 .. n-1 in order.  The neighbour indices are read four at a
+// time, one group ahead, and the gather of neighbour k + 1 is issued before the arithmetic of
+// neighbour k: no gather waits on its own index, and one gather is always in flight.
+// Rows are 16-byte aligned and hold a multiple of 4 (at least 8) entries; the entries past n
+// that the vector reads pick up are never dereferenced.
+template <class Load, class Body>
+__device__ __forceinline__ void gather_loop(const int *row, int n, Load &&load, Body &&body) {
+  if (n <= 0) return;
+  typedef int iv4 __attribute__((ext_vector_type(4)));
+  iv4 cur = *(const iv4 *)row;
+  iv4 nxt = n > 4 ? *(const iv4 *)(row + 4) : cur;
+  auto ra = load(cur.x);
+  decltype(ra) rb;
+  for (int k = 0; k < n; k += 4) {
+    if (k + 1 < n) rb = load(cur.y);
+    body(ra);
+    if (k + 1 >= n) break;
+    if (k + 2 < n) ra = load(cur.z);
+    body(rb);
+    if (k + 2 >= n) break;
+    if (k + 3 < n) rb = load(cur.w);
+    body(ra);
+    if (k + 3 >= n) break;
+    cur = nxt;
+    if (k + 8 < n) nxt = *(const iv4 *)(row + k + 8);
+    if (k + 4 < n) ra = load(cur.x);
+    body(rb);
   }
 }
 
@@ -242,10 +283,10 @@ __device__ __forceinline__ void particle_update(const State &st, unsigned i, uin
 
 // ---------------------------------------------------------------------------------------------
 // FP64 pair pass
-__global__ void __launch_bounds__(BSP)
+__global__ void __launch_bounds__(BSP, PAIR64_WAVES)
     k_pair64(unsigned n_init, State st, const Rec64 *__restrict__ rec, const int *__restrict__ nbr,
              int stride, const int *__restrict__ count, Consts c, const Counters *cnt) {
-  const unsigned i = blockIdx.x * BSP + threadIdx.x;
+  const unsigned i = xcd_block() * BSP + threadIdx.x;
   const unsigned N = cnt->N;
   if (i >= N || i >= n_init) return;
   const uint32_t fl = st.flags[i];
@@ -259,6 +300,7 @@ __global__ void __launch_bounds__(BSP)
   const double h1 = 1 / hi;
   const double fac = (INV_PI_D)*h1 * h1 * h1;
   const int n = count[i];
+  const int *row = nbr + (size_t)i * stride;
   const int single = c.single_sqrt;
   const double mass = c.i_mass;
 
@@ -274,19 +316,21 @@ __global__ void __launch_bounds__(BSP)
 #pragma unroll
       for (int b = 0; b < 3; b++) B.m[a][b] = 0.;
     if (!tool_nz) {
-      for (int k = 0; k < n; k++) {
-        const int j = nbr[(size_t)k * stride + i];
-        const double4 pj = *(const double4 *)&rec[j].x;
-        const double volj = rec[j].vol, tj = rec[j].tool;
-        if (tj == 1.) continue;
-        double xij = ri.x - pj.x, yij = ri.y - pj.y, zij = ri.z - pj.z;
+      struct G {
+        double4 p, q;  // (x, y, z, h), (T, vol, c, tool)
+      };
+      auto load = [&](int j) { return G{*(const double4 *)&rec[j].x, *(const double4 *)&rec[j].T}; };
+      gather_loop(row, n, load, [&](const G &g) {
+        if (g.q.w == 1.) return;
+        const double volj = g.q.y;
+        double xij = ri.x - g.p.x, yij = ri.y - g.p.y, zij = ri.z - g.p.z;
         double rr2 = xij * xij + yij * yij + zij * zij;
         double w, wx, wy, wz;
         cubic_spline64(xij, yij, zij, rr2, h1, fac, single, w, wx, wy, wz);
         B.m[0][0] -= volj * xij * wx; B.m[1][0] -= volj * xij * wy; B.m[2][0] -= volj * xij * wz;
         B.m[0][1] -= volj * yij * wx; B.m[1][1] -= volj * yij * wy; B.m[2][1] -= volj * yij * wz;
         B.m[0][2] -= volj * zij * wx; B.m[1][2] -= volj * zij * wy; B.m[2][2] -= volj * zij * wz;
-      }
+      });
     }
     if (det3(B) > 1e-8) inv3(B, invB);
   }
@@ -305,9 +349,7 @@ __global__ void __launch_bounds__(BSP)
   ps.Tl = 0.;
   const double Ti = brook ? ri.T : 0.;
   const double wdeltap = c.wdeltap, eps = c.xspheps;
-  for (int k = 0; k < n; k++) {
-    const int j = nbr[(size_t)k * stride + i];
-    const Rec64 rj = rec[j];
+  auto body = [&](const Rec64 &rj) {
     double xij = ri.x - rj.x, yij = ri.y - rj.y, zij = ri.z - rj.z;
     double rr2 = xij * xij + yij * yij + zij * zij;
     double w, wx0, wy0, wz0;
@@ -372,14 +414,27 @@ __global__ void __launch_bounds__(BSP)
         ps.Tl += 2.0 * (rj.vol) * (Ti - rj.T) * rij1 * (ex * w_x + ey * w_y + ez * w_z);
       }
     }
-  }
+  };
+  // two record buffers alternate: the gather of neighbour k + 1 is in flight during the
+  // arithmetic of neighbour k (the sums still run in neighbour-list order)
+#if PAIR64_PIPELINE
+  gather_loop(row, n, [&](int j) { return rec[j]; }, body);
+#else
+  for (int k = 0; k < n; k++) body(rec[row[k]]);
+#endif
   particle_update(st, i, fl, c, ps, brook, thermal_alpha);
 }
 
 // ---------------------------------------------------------------------------------------------
-// FP32 pair pass with Kahan-compensated neighbour sums
-struct KF {
+// FP32 pair pass.  Positions stay FP64 until the relative position is formed; the pair
+// arithmetic is FP32 (contracted to FMA); every neighbour sum is compensated, either
+//   AccKahan  FP32 running sum with Kahan's correction term (4 operations per term), or
+//   AccWide   the FP32 term widened and added in FP64 (2 operations per term: on CDNA4 an FP64
+//             add issues at the FP32 rate; the summation error, n 2^-53 sum|t|, is below
+//             Kahan's 2 2^-24 sum|t|).
+struct AccKahan {
   float s, e;
+  __device__ __forceinline__ void zero() { s = e = 0.f; }
   __device__ __forceinline__ void add(float v) {
     float y = v - e;
     float t = s + y;
@@ -388,11 +443,23 @@ struct KF {
   }
   __device__ __forceinline__ double val() const { return (double)s - (double)e; }
 };
+struct AccWide {
+  double s;
+  __device__ __forceinline__ void zero() { s = 0.; }
+  __device__ __forceinline__ void add(float v) { s += (double)v; }
+  __device__ __forceinline__ double val() const { return s; }
+};
 
-__global__ void __launch_bounds__(BSP)
+struct ToolVol {
+  float tool, vol;
+};
+
+template <class Acc>
+__global__ void __launch_bounds__(BSP, PAIR32_WAVES)
     k_pair32(unsigned n_init, State st, const Rec32 *__restrict__ rec, const int *__restrict__ nbr,
              int stride, const int *__restrict__ count, Consts c, const Counters *cnt) {
-  const unsigned i = blockIdx.x * BSP + threadIdx.x;
+#pragma clang fp contract(fast)
+  const unsigned i = xcd_block() * BSP + threadIdx.x;
   const unsigned N = cnt->N;
   if (i >= N || i >= n_init) return;
   const uint32_t fl = st.flags[i];
@@ -406,31 +473,32 @@ __global__ void __launch_bounds__(BSP)
   const float h1 = __builtin_amdgcn_rcpf(hi);
   const float fac = (float)INV_PI_D * h1 * h1 * h1;
   const int n = count[i];
+  const int *row = nbr + (size_t)i * stride;
   const float mass = (float)c.i_mass;
 
   float iB[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};  // invB[col][row] flattened
   if (c.cspm) {
-    KF B[9];
+    Acc B[9];
 #pragma unroll
-    for (int q = 0; q < 9; q++) B[q] = KF{0.f, 0.f};
+    for (int q = 0; q < 9; q++) B[q].zero();
     if (!tool_nz) {
-      for (int k = 0; k < n; k++) {
-        const int j = nbr[(size_t)k * stride + i];
-        const Rec32 &rj = rec[j];
-        if (rj.tool == 1.f) continue;
-        float xij = (float)(ri.x - rj.x), yij = (float)(ri.y - rj.y), zij = (float)(ri.z - rj.z);
+      // x, y, z, (tool, vol) are the first 32 bytes of a record
+      gather_loop(row, n, [&](int j) { return *(const double4 *)&rec[j].x; }, [&](const double4 &g) {
+        const ToolVol tv = __builtin_bit_cast(ToolVol, g.w);
+        if (tv.tool == 1.f) return;
+        float xij = (float)(ri.x - g.x), yij = (float)(ri.y - g.y), zij = (float)(ri.z - g.z);
         float rr2 = xij * xij + yij * yij + zij * zij;
         float rad = __builtin_amdgcn_sqrtf(rr2);
         float rinv = __builtin_amdgcn_rcpf(rad);
         float q = rad * h1;
         float der = (q > 1.f) ? -0.75f * (2.f - q) * (2.f - q) * h1 * rinv
                               : -3.0f * q * (1.f - 0.75f * q) * h1 * rinv;
-        float dw = der * fac * rj.vol;
+        float dw = der * fac * tv.vol;
         float gx = dw * xij, gy = dw * yij, gz = dw * zij;
         B[0].add(-xij * gx); B[3].add(-xij * gy); B[6].add(-xij * gz);
         B[1].add(-yij * gx); B[4].add(-yij * gy); B[7].add(-yij * gz);
         B[2].add(-zij * gx); B[5].add(-zij * gy); B[8].add(-zij * gz);
-      }
+      });
     }
     M3 Bd, inv;
 #pragma unroll
@@ -446,25 +514,23 @@ __global__ void __launch_bounds__(BSP)
     }
   }
 
-  KF vd[9], sd[9], vt[3], xt[3], Tl;
+  Acc vd[9], sd[9], vt[3], xt[3], Tl;
 #pragma unroll
   for (int q = 0; q < 9; q++) {
-    vd[q] = KF{0.f, 0.f};
-    sd[q] = KF{0.f, 0.f};
+    vd[q].zero();
+    sd[q].zero();
   }
 #pragma unroll
   for (int a = 0; a < 3; a++) {
-    vt[a] = KF{0.f, 0.f};
-    xt[a] = KF{0.f, 0.f};
+    vt[a].zero();
+    xt[a].zero();
   }
-  Tl = KF{0.f, 0.f};
+  Tl.zero();
   const float Ti = brook ? ri.T : 0.f;
   const float inv_wdp = c.wdeltap > 0 ? (float)(1.0 / c.wdeltap) : 0.f;
   const float eps = (float)c.xspheps, va = (float)c.v_alpha, vb = (float)c.v_beta;
   const float ve2 = (float)(c.v_eta * c.v_eta);
-  for (int k = 0; k < n; k++) {
-    const int j = nbr[(size_t)k * stride + i];
-    const Rec32 rj = rec[j];
+  auto body = [&](const Rec32 &rj) {
     float xij = (float)(ri.x - rj.x), yij = (float)(ri.y - rj.y), zij = (float)(ri.z - rj.z);
     float rr2 = xij * xij + yij * yij + zij * zij;
     float rad = __builtin_amdgcn_sqrtf(rr2);
@@ -525,7 +591,8 @@ __global__ void __launch_bounds__(BSP)
       float e_w = (xij * w_x + yij * w_y + zij * w_z) * rinv;
       Tl.add(2.0f * rj.vol * (Ti - rj.T) * rinv * e_w);
     }
-  }
+  };
+  gather_loop(row, n, [&](int j) { return rec[j]; }, body);
   PairSums ps;
 #pragma unroll
   for (int a = 0; a < 3; a++)
@@ -561,8 +628,11 @@ void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *re
   if (c.precision == MFG_FP64)
     hipLaunchKernelGGL(k_pair64, dim3(nb(n_init, BSP)), dim3(BSP), 0, s, n_init, st,
                        (const Rec64 *)rec, nbr, nl_stride, count, c, cnt);
+  else if (c.precision == MFG_FP32_KAHAN)
+    hipLaunchKernelGGL(k_pair32<AccKahan>, dim3(nb(n_init, BSP)), dim3(BSP), 0, s, n_init, st,
+                       (const Rec32 *)rec, nbr, nl_stride, count, c, cnt);
   else
-    hipLaunchKernelGGL(k_pair32, dim3(nb(n_init, BSP)), dim3(BSP), 0, s, n_init, st,
+    hipLaunchKernelGGL(k_pair32<AccWide>, dim3(nb(n_init, BSP)), dim3(BSP), 0, s, n_init, st,
                        (const Rec32 *)rec, nbr, nl_stride, count, c, cnt);
 }
 

@@ -11,6 +11,21 @@
 
 namespace mfg {
 
+// Workgroups of one launch are handed to the 8 XCDs of the chip round robin (workgroup b runs on
+// XCD b % 8), and every XCD has its own L2.  Renumbering them so that each XCD sweeps one
+// contiguous range of the sorted particles keeps the neighbour data of spatially close particles
+// in the same L2.  The map is a bijection of [0, gridDim.x).
+#ifndef MFG_XCD_REMAP
+#define MFG_XCD_REMAP 1
+#endif
+__device__ __forceinline__ unsigned xcd_block() {
+  if (!MFG_XCD_REMAP) return blockIdx.x;
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  const unsigned x = b & 7u, q = nb >> 3, r = nb & 7u;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+
 // Correctly rounded square roots.  HIP's __fsqrt_rn maps to the approximate hardware sqrt
 // unless OCML_BASIC_ROUNDED_OPERATIONS is defined; the builtins are IEEE-rounded under the
 // default -fhip-fp32-correctly-rounded-divide-sqrt.

@@ -9,7 +9,7 @@
 //     nl4   : double4 (x, y, z, blanked)          -- candidate filter of the neighbour-list pass
 //     pse2  : double2 (T, rho)                    -- PSE heat pass (only with PSE)
 //     rec   : pair record, 192 B (FP64) / 112 B (FP32) -- everything a neighbour contributes
-//   neighbour list: int32 nbr[k * nl_stride + i], count[i]
+//   neighbour list: int32 nbr[i * cap + k] (row-major), count[i]
 //   grid: key[N_init] (unsorted), key_sorted[N_init], order[N_init], cell counts / offsets
 //         [num_cell + 1], fill counters [num_cell]
 #pragma once
@@ -60,14 +60,15 @@ struct alignas(16) Rec64 {
   double T, vol, c, tool;
 };
 
-// pair record, FP32 pair arithmetic: positions stay FP64
+// pair record, FP32 pair arithmetic: positions stay FP64; the first 32 bytes hold everything
+// the CSPM pass reads (x, y, z, tool, vol)
 struct alignas(16) Rec32 {
   double x, y, z;
-  float h, vx, vy, vz, rho;
+  float tool, vol;
+  float h, vx, vy, vz;
+  float rho, T, c, pad;
   float A[6];
   float R[6];
-  float T, vol, c, tool;
-  float pad;
 };
 
 static_assert(sizeof(Rec64) == 192, "Rec64 layout");

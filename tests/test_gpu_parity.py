@@ -177,14 +177,16 @@ def test_thermal_contact_cut_fp64(hip_device):
     print("thermal contact fp64 relative errors", errs)
 
 
-def test_fp32_kahan_against_fp64_oracle(hip_device):
-    """FP32 pair arithmetic with Kahan sums vs the FP64 oracle: relative L2 error of stress,
-    temperature, plastic strain and position below 1e-4 after the wedge cut engages."""
+@pytest.mark.parametrize("precision", [_lib.MFG_FP32_KAHAN, _lib.MFG_FP32_WIDE], ids=["kahan", "wide"])
+def test_fp32_against_fp64_oracle(hip_device, precision):
+    """FP32 pair arithmetic with compensated sums (Kahan, or widened to FP64) vs the FP64 oracle:
+    relative L2 error of stress, temperature and position below 1e-4 after 200 steps of the
+    wedge cut."""
     sc = scenes.Scene(1, 1.0, 1)
     sim = helpers.oracle_from_scene(sc)
     dt = sc.params.dt
     sim.set_time(dt, dt)
-    parts, grid = scenes.setup_cut(sc, precision=_lib.MFG_FP32_KAHAN)
+    parts, grid = scenes.setup_cut(sc, precision=precision)
     api.global_time_current = dt
     lf = api.leap_frog(sc.n, grid.num_cell())
     n = 200
@@ -202,6 +204,6 @@ def test_fp32_kahan_against_fp64_oracle(hip_device):
         if f == O.F_S:
             o = 0.5 * (o + o.reshape(-1, 3, 3).transpose(0, 2, 1).reshape(-1, 9))
         errs[name] = rel_err(g, o)
-    print("fp32+kahan relative errors vs fp64 oracle", errs)
+    print(f"fp32 (precision {precision}) relative errors vs fp64 oracle", errs)
     for name in ("S", "T", "pos"):
         assert errs[name] < 1e-4, errs
