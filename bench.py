@@ -16,6 +16,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -36,6 +37,14 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3
 FLOPS_PER_PAIR = 280.0
 FLOPS_PER_CANDIDATE = 11.0
 FLOPS_PER_PSE_CANDIDATE = 10.0
+
+
+WORKLOADS = {
+    (1, 1.0): "orthogonal cut, 8,000-particle block (40x20x10, dx 10 um), 12-triangle wedge, Brookshaw heat",
+    (2, 1.0): "orthogonal cut, 1M-particle block (200x100x50, dx 5 um), single synthetic grain "
+              "(4,800 triangles), PSE heat with tool-workpiece thermal contact",
+    (3, 1.0): "orthogonal cut, 8M-particle block (400x200x100, dx 2.5 um), three synthetic grains",
+}
 
 
 def log(msg: str) -> None:
@@ -76,24 +85,64 @@ def gpu_run(args):
                 parts=parts, n_particles=scene.n)
 
 
-def roofline(res, precision: str, pse: bool):
+# algorithmic HBM bytes of one pair pass per particle, besides 4 B per neighbour-list entry:
+#   own pair record (192 B FP64 / 112 B FP32) + count 4 B + flags 4 B
+#   + start-of-step state read 264 B (x, v, S, S_t, v_t, rho, T, rho_t, T_t, fc, ft, eps_pl,
+#     eps_pl_dot) + new state and rates written 240 B
+PAIR_BYTES_PER_PARTICLE = {"fp64": 704.0, "fp32": 624.0}
+
+# committed rocprofv3 FETCH_SIZE / WRITE_SIZE summaries: (kernel, preset, scale) -> file
+TRAFFIC_PROFILES = {
+    ("pair_update_fp64", 2, 1.0): "pair_update_fp64_cut1M_single_grain_traffic.md",
+}
+
+
+def measured_traffic(kernel: str, args):
+    """HBM-side bytes per launch of `kernel` from the committed profiler summary of this workload
+    (FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known size), or None."""
+    name = TRAFFIC_PROFILES.get((kernel, args.preset, float(args.scale)))
+    if name is None:
+        return None
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch: (\d+)", open(path).read())
+    return int(m.group(1)) if m else None
+
+
+def roofline(res, args, pse: bool):
+    """Roofline of the dominant kernel.  Its bound is arithmetic: the pair pass runs on the vector
+    ALUs (no step of the method is a dense contraction, so no matrix instructions), and on MI355X
+    the FP64 vector and matrix peaks are the same 78.6 TFLOP/s.  The HBM view (algorithmic bytes
+    over the launch time, and the measured traffic) is reported beside it."""
     kernels = res["kernels"]
     dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
     avg_s = kernels[dom]["avg_ms"] * 1e-3
-    peak = FP64_VECTOR_PEAK_TFLOPS if precision == "fp64" else FP32_VECTOR_PEAK_TFLOPS
+    fp64 = args.precision == "fp64"
+    peak = FP64_VECTOR_PEAK_TFLOPS if fp64 else FP32_VECTOR_PEAK_TFLOPS
+    n = res["n_active"]
     if dom.startswith("pair_update"):
         flops = FLOPS_PER_PAIR * res["pairs"]
         basis = f"{FLOPS_PER_PAIR:.0f} flop x {res['pairs']} interacting pairs per launch"
+        alg_bytes = PAIR_BYTES_PER_PARTICLE["fp64" if fp64 else "fp32"] * n + 4.0 * res["pairs"]
     elif dom == "neighbours":
         per = FLOPS_PER_CANDIDATE + (FLOPS_PER_PSE_CANDIDATE if pse else 0.0)
         flops = per * res["cand"]
         basis = f"{per:.0f} flop x {res['cand']} candidate pairs per launch"
+        alg_bytes = 76.0 * n + 4.0 * res["pairs"]
     else:
         return dom, None
     achieved = flops / avg_s / 1e12
-    return dom, {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                 "frac": achieved / peak, "traffic": None, "kernel": dom, "basis": basis,
-                 "avg_launch_ms": kernels[dom]["avg_ms"]}
+    hbm = alg_bytes / avg_s / 1e9
+    return dom, {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                 "frac": achieved / peak, "traffic": measured_traffic(dom, args), "kernel": dom,
+                 "basis": basis, "avg_launch_ms": kernels[dom]["avg_ms"],
+                 "compute_units": ("FP64 vector ALU; no matrix instructions (the FP64 vector and matrix peaks "
+                                   "of MI355X are both 78.6 TFLOP/s)" if fp64 else
+                                   "FP32 vector ALU, packed FP32 not generated; peak quoted is the published "
+                                   "157.3 TFLOP/s FP32 figure"),
+                 "algorithmic_bytes": alg_bytes, "hbm_achieved_GBps": hbm, "hbm_peak_GBps": HBM_PEAK_GBS,
+                 "hbm_frac": hbm / HBM_PEAK_GBS}
 
 
 def cpu_baseline(args):
@@ -139,7 +188,9 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--precision", choices=["fp64", "fp32-kahan", "fp32-wide"], default="fp64")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=10,
+                    help="steps of the CPU oracle timed for cpu_baseline (about 1 s each at 1M particles "
+                         "on 16 threads)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain copy of this many MiB (profiler counter calibration)")
@@ -149,12 +200,12 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the step shards by x slabs, see DESIGN.md)", file=sys.stderr)
+              "(one process drives one GPU; the step would shard by slabs along the cut direction)", file=sys.stderr)
         sys.exit(2)
     res = gpu_run(args)
     scene = res["scene"]
     pse = scene.params.heat == 2
-    dom, roof = roofline(res, args.precision, pse)
+    dom, roof = roofline(res, args, pse)
     for k, v in sorted(res["kernels"].items(), key=lambda kv: -kv[1]["total_ms"]):
         log(f"  {k:20s} {v['launches']:6d} launches  avg {v['avg_ms']:.4f} ms  total {v['total_ms']:.1f} ms")
     log(f"GPU: {res['value']:.4e} particle-steps/s over {args.steps} steps ({res['secs']:.3f} s)")
@@ -163,7 +214,7 @@ def main():
         cpu = cpu_baseline(args)
         log(f"CPU oracle: {cpu['value']:.4e} particle-steps/s on {cpu['cores']} threads")
     out = {
-        "metric": "particle-steps/s at 1M particles (FP64 method, single grain, PSE + thermal contact)",
+        "metric": "particle-steps/s at 1M/8M particles, FP32 vs FP64; p99 sweep latency",
         "value": res["value"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -173,9 +224,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32",
         "data": "synthetic (seeded lattice + jitter, synthetic grain mesh)",
-        "config": {"workload": f"orthogonal cut preset {args.preset} (scale {args.scale})",
+        "config": {"workload": WORKLOADS.get((args.preset, float(args.scale)),
+                                             f"orthogonal cut preset {args.preset} (scale {args.scale})"),
                    "particles": res["n_particles"], "active_particles": res["n_active"],
                    "precision": args.precision, "seed": args.seed,
                    "pairs_per_particle": res["pairs"] / max(res["n_active"], 1),
