@@ -183,6 +183,8 @@ struct mfg_particles {
   int contact_blocks = 0;
   double *p_dbg = nullptr, *R_dbg = nullptr;
   int *flag_d = nullptr;
+  unsigned long long *stat_part = nullptr;  // per-workgroup (candidates, pairs) of the last list build
+  int stat_cap = 0, stat_n = 0;
   const State &st() const { return buf[cur]; }
 };
 
@@ -265,7 +267,7 @@ size_t padded(size_t n) { return (n + 63) / 64 * 64; }
 void free_particles(mfg_particles *p) {
   void *ptrs[] = {p->arena[0], p->arena[1], p->uarena[0], p->uarena[1], p->rec, p->nl4, p->pse2,
                   p->nbr, p->count, p->key, p->key_sorted, p->order, p->blk, p->cnt, p->partials,
-                  p->p_dbg, p->R_dbg, p->flag_d};
+                  p->p_dbg, p->R_dbg, p->flag_d, p->stat_part};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (p->cnt_h) (void)hipHostFree(p->cnt_h);
@@ -376,6 +378,25 @@ void permute(mfg_grid *g, mfg_particles *p, const Consts &c, int predict) {
   launch_permute_predict_ex(g_lib.stream, p->buf[p->cur], p->buf[nxt], p->N_init, gb, p->cnt, c,
                             predict, p->rec, p->nl4, p->pse2, p->p_dbg, p->R_dbg);
   p->cur = nxt;
+}
+
+// neighbour lists (+ PSE) of the current records on grid g
+int run_neighbours(mfg_grid *g, mfg_particles *p, const Consts &c, int want_pse) {
+  const GridInfo gi = g->info();
+  const int nb = neighbour_blocks(gi);
+  if (nb > p->stat_cap) {
+    if (p->stat_part) (void)hipFree(p->stat_part);
+    p->stat_part = nullptr;
+    p->stat_cap = 0;
+    if (hipMalloc(&p->stat_part, sizeof(unsigned long long) * 2 * (size_t)nb) != hipSuccess)
+      return fail(MFG_ERR_NOMEM, "neighbour statistics (%d workgroups)", nb);
+    p->stat_cap = nb;
+  }
+  p->stat_n = nb;
+  launch_neighbours(g_lib.stream, gi, g->count, p->nl4, p->pse2, p->st().h, p->st().flags, c,
+                    p->nbr, p->nl_stride, p->count, p->st().T_t, p->cnt, want_pse, p->stat_part);
+  HIP_TRY(hipGetLastError());
+  return MFG_OK;
 }
 
 int check_counters(mfg_particles *p) {
@@ -742,8 +763,17 @@ int mfg_particle_gpu_pair_stats(mfg_particles *p, unsigned long long *cand,
   NEED_LIB();
   if (!p) return fail(MFG_ERR_ARG, "null particles");
   int rc = check_counters(p);
-  if (cand) *cand = p->cnt_h->candidates;
-  if (pairs) *pairs = p->cnt_h->pairs;
+  unsigned long long a = 0, b = 0;
+  if (p->stat_n > 0) {
+    std::vector<unsigned long long> h(2 * (size_t)p->stat_n);
+    HIP_TRY(hipMemcpy(h.data(), p->stat_part, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < p->stat_n; k++) {
+      a += h[2 * k];
+      b += h[2 * k + 1];
+    }
+  }
+  if (cand) *cand = a;
+  if (pairs) *pairs = b;
   return rc;
 }
 
@@ -1112,9 +1142,8 @@ int mfg_grid_neighbours(mfg_grid *g, mfg_particles *p, int *h_counts, int *h_lis
   p->cur = 1 - p->cur;
   HIP_TRY(hipMemcpyAsync(p->order, saved_order, sizeof(int) * p->N_init, hipMemcpyDeviceToDevice, g_lib.stream));
   HIP_TRY(hipMemcpyAsync(p->key, saved_keys, sizeof(int) * p->N_init, hipMemcpyDeviceToDevice, g_lib.stream));
-  launch_neighbours(g_lib.stream, p->N_init, g->info(), g->count, p->key_sorted, p->nl4, p->pse2,
-                    p->st().h, p->st().flags, c, p->nbr, p->nl_stride, p->cap, p->count,
-                    p->st().T_t, p->cnt, 0);
+  int rn = run_neighbours(g, p, c, 0);
+  if (rn != MFG_OK) return rn;
   HIP_TRY(hipStreamSynchronize(g_lib.stream));
   (void)hipFree(saved_order);
   (void)hipFree(saved_keys);
@@ -1447,9 +1476,8 @@ int mfg_leap_frog_step(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int rec
   const bool pse = c.heat == MFG_HEAT_PSE && c.pse_on;
   {
     ProfScope ps("neighbours");
-    launch_neighbours(s, p->N_init, g->info(), g->count, p->key_sorted, p->nl4, p->pse2,
-                      p->st().h, p->st().flags, c, p->nbr, p->nl_stride, p->cap, p->count,
-                      p->st().T_t, p->cnt, pse ? 1 : 0);
+    rc = run_neighbours(g, p, c, pse ? 1 : 0);
+    if (rc != MFG_OK) return rc;
   }
   {
     ProfScope ps(p->precision == MFG_FP64          ? "pair_update_fp64"

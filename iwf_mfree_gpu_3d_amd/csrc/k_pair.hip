@@ -39,88 +39,160 @@ constexpr double PI_D = 3.14159265358979323846;
 constexpr double INV_PI_D = 0.31830988618379067154;
 
 // ---------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(BSP)
-    k_neighbours(unsigned n_init, GridInfo g, const int *off, const int *key_sorted,
-                 const double4 *nl4, const double2 *pse2, const double *h, const uint32_t *flags,
-                 Consts c, int *nbr, int stride, int cap, int *count, double *T_t, Counters *cnt,
-                 int want_pse) {
-  __shared__ unsigned long long red[2][BSP / 64];
-  const unsigned i = xcd_block() * BSP + threadIdx.x;
-  const unsigned N = cnt->N;
+// One wavefront per cell.  All particles of a cell share the 25 candidate ranges, so the wave
+// lists the candidates once (in LDS, in the reference's visiting order) and its lanes take
+// candidates: the candidate loads are coalesced and no lane idles on a shorter range.  For each
+// particle of the cell, in sorted order, an ordered ballot compaction appends the kept
+// candidates to the particle's row-major list, and the PSE sum is a per-lane partial reduced
+// over the wave at the end.
+//   PSE_MODE 0: no PSE sum; 1: FP64 PSE terms; 2: FP32 PSE terms accumulated in FP64
+constexpr int NB_WAVES = 4;             // cells (wavefronts) per workgroup
+constexpr int NB_BLOCK = 64 * NB_WAVES;
+constexpr int NB_LCAP = 1024;           // candidates listed in LDS per cell (beyond: searched)
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int PSE_MODE>
+__global__ void __launch_bounds__(NB_BLOCK)
+    k_cell_neighbours(GridInfo g, const int *__restrict__ off, const double4 *__restrict__ nl4,
+                      const double2 *__restrict__ pse2, const double *__restrict__ h,
+                      const uint32_t *__restrict__ flags, Consts c, int *__restrict__ nbr, int cap,
+                      int *__restrict__ count, double *__restrict__ T_t, Counters *cnt,
+                      unsigned long long *__restrict__ stat_part) {
+  __shared__ int s_cand[NB_WAVES][NB_LCAP];
+  __shared__ int s_pref[NB_WAVES][32];
+  __shared__ int s_rst[NB_WAVES][32];
+  __shared__ unsigned long long s_stat[NB_WAVES][2];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long cell = (long)xcd_block() * NB_WAVES + w;
   unsigned long long n_cand = 0, n_pair = 0;
-  if (i < N && i < n_init) {
-  const double4 pi = nl4[i];
-  const bool blanked_i = pi.w != 0.;
-  const double hi = h[i];
-  const double fourh2 = 4 * hi * hi;
-  const int key = key_sorted[i];
-  const int nyz = g.ny * g.nz;
-  const int gi = key / nyz;
-  const int gj = (key - gi * nyz) / g.nz;
-  const int gk = key % g.nz;
-  const int lo_i = gi - 2 < 0 ? 0 : gi - 2, hi_i = gi + 3 > g.nx ? g.nx : gi + 3;
-  const int lo_j = gj - 2 < 0 ? 0 : gj - 2, hi_j = gj + 3 > g.ny ? g.ny : gj + 3;
-  const int lo_k = gk - 2 < 0 ? 0 : gk - 2, hi_k = gk + 3 > g.nz ? g.nz : gk + 3;
-  double Ti = 0., coef = 0., h2 = 0., Tti = 0.;
-  if (want_pse) {
-    Ti = pse2[i].x;
-    h2 = hi * hi;
-    double h4 = h2 * h2;
-    double h5 = hi * h4;
-    coef = +4. / (h5 * PI_D * sqrt_rn(PI_D));
-  }
-  int n = 0;
-  for (int ii = lo_i; ii < hi_i; ii++)
-    for (int jj = lo_j; jj < hi_j; jj++) {
-      const int cb = ii * nyz + jj * g.nz;
-      const int s = off[cb + lo_k], e = off[cb + hi_k];
-      n_cand += (unsigned long long)(e - s);
-      for (int j = s; j < e; j++) {
-        const double4 pj = nl4[j];
-        double xij = pi.x - pj.x, yij = pi.y - pj.y, zij = pi.z - pj.z;
-        double rr2 = xij * xij + yij * yij + zij * zij;
-        if (want_pse) {
-          const double2 tr = pse2[j];
-          double xx = sqrt_rn(rr2);
-          double w2 = coef * exp(-xx * xx / (h2));
-          Tti += (tr.x - Ti) * w2 * c.i_mass / tr.y;
-        }
-        if (!blanked_i && pj.w == 0. && !(rr2 >= fourh2 || rr2 < 1e-8)) {
-          if (n < cap) nbr[(size_t)i * stride + n] = j;
-          n++;
-        }
-      }
+  const int s = cell < g.num_cell ? off[cell] : 0;
+  const int e = cell < g.num_cell ? off[cell + 1] : 0;
+  if (s < e) {
+    const int nyz = g.ny * g.nz;
+    const int gi = (int)(cell / nyz);
+    const int gj = (int)((cell - (long)gi * nyz) / g.nz);
+    const int gk = (int)(cell % g.nz);
+    const int lo_i = gi - 2 < 0 ? 0 : gi - 2, hi_i = gi + 3 > g.nx ? g.nx : gi + 3;
+    const int lo_j = gj - 2 < 0 ? 0 : gj - 2, hi_j = gj + 3 > g.ny ? g.ny : gj + 3;
+    const int lo_k = gk - 2 < 0 ? 0 : gk - 2, hi_k = gk + 3 > g.nz ? g.nz : gk + 3;
+    const int nj = hi_j - lo_j, nr = (hi_i - lo_i) * nj;
+    // candidate range of stencil column (ii, jj) = lane, its exclusive prefix over the columns
+    int st = 0, len = 0;
+    if (lane < nr) {
+      const int ii = lo_i + lane / nj, jj = lo_j + lane % nj;
+      const long cb = (long)ii * nyz + (long)jj * g.nz;
+      st = off[cb + lo_k];
+      len = off[cb + hi_k] - st;
     }
-  if (n > cap) {
-    atomicOr(&cnt->err_capacity, 1);
-    atomicMax(&cnt->max_count, n);
-    n = cap;
-  } else {
-    atomicMax(&cnt->max_count, n);
+    int incl = len;
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+      const int t = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += t;
+    }
+    const int T = __builtin_amdgcn_readfirstlane(__shfl(incl, 31, 64));
+    if (lane < 32) {
+      s_pref[w][lane] = incl - len;  // columns >= nr: T
+      s_rst[w][lane] = st;
+    }
+    wave_lds_sync();
+    const bool in_lds = T <= NB_LCAP;
+    if (in_lds) {
+      for (int r = 0; r < nr; r++) {
+        const int rs = s_rst[w][r], rp = s_pref[w][r], rl = s_pref[w][r + 1] - rp;
+        for (int t = lane; t < rl; t += 64) s_cand[w][rp + t] = rs + t;
+      }
+      wave_lds_sync();
+    }
+    auto cand_at = [&](int q) -> int {
+      if (in_lds) return s_cand[w][q];
+      int lo = 0, hi = nr - 1;  // last column whose prefix is <= q (never an empty one)
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_pref[w][mid] <= q) lo = mid;
+        else hi = mid - 1;
+      }
+      return s_rst[w][lo] + (q - s_pref[w][lo]);
+    };
+    for (int p = s; p < e; p++) {
+      const double4 pi = nl4[p];
+      const bool blanked_i = pi.w != 0.;
+      const double hi = h[p];
+      const double fourh2 = 4 * hi * hi;
+      double Ti = 0., coef = 0., h2 = 0.;
+      if (PSE_MODE) {
+        Ti = pse2[p].x;
+        h2 = hi * hi;
+        const double h4 = h2 * h2;
+        const double h5 = hi * h4;
+        coef = +4. / (h5 * PI_D * sqrt_rn(PI_D));
+      }
+      const float coef_f = (float)coef, inv_h2_f = (float)(1. / h2), mass_f = (float)c.i_mass;
+      int np = 0;
+      double acc = 0.;
+      int *out = nbr + (size_t)p * cap;
+      for (int b0 = 0; b0 < T; b0 += 64) {
+        const int q = b0 + lane;
+        const bool valid = q < T;
+        const int j = valid ? cand_at(q) : p;
+        const double4 pj = nl4[j];
+        const double xij = pi.x - pj.x, yij = pi.y - pj.y, zij = pi.z - pj.z;
+        const double rr2 = xij * xij + yij * yij + zij * zij;
+        if (PSE_MODE == 1 && valid) {
+          // lapl_pse (src/kernels.cuh:71-92) in the order of src/interactions_gpu.cu:124-147
+          const double2 tr = pse2[j];
+          const double xx = sqrt_rn(rr2);
+          const double w2 = coef * exp(-xx * xx / (h2));
+          acc += (tr.x - Ti) * w2 * c.i_mass / tr.y;
+        }
+        if (PSE_MODE == 2 && valid) {
+          const double2 tr = pse2[j];
+          const float w2 = coef_f * __expf(-(float)rr2 * inv_h2_f);
+          acc += (double)((float)(tr.x - Ti) * w2 * mass_f * __builtin_amdgcn_rcpf((float)tr.y));
+        }
+        const bool ok = valid && !blanked_i && pj.w == 0. && !(rr2 >= fourh2 || rr2 < 1e-8);
+        const unsigned long long m = __ballot(ok);
+        const int pos = np + __popcll(m & ((1ull << lane) - 1ull));
+        if (ok && pos < cap) out[pos] = j;
+        np += __popcll(m);
+      }
+      if (PSE_MODE) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+      }
+      if (lane == 0) {
+        if (np > cap) {
+          atomicOr(&cnt->err_capacity, 1);
+          atomicMax(&cnt->max_count, np);
+        }
+        count[p] = np > cap ? cap : np;
+        if (PSE_MODE) T_t[p] = ((flags[p] & FL_TOOL) ? c.alpha_tool : c.alpha_wp) * acc;
+      }
+      n_cand += (unsigned long long)T;
+      n_pair += (unsigned long long)(np > cap ? cap : np);
+    }
   }
-  count[i] = n;
-  if (want_pse) T_t[i] = ((flags[i] & FL_TOOL) ? c.alpha_tool : c.alpha_wp) * Tti;
-  n_pair = (unsigned long long)n;
-  }
-  // candidate / pair totals of this pass (work accounting of the roofline figures)
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int o = 32; o > 0; o >>= 1) {
-    n_cand += __shfl_down(n_cand, o, 64);
-    n_pair += __shfl_down(n_pair, o, 64);
-  }
+  // candidate / pair totals of this workgroup (work accounting of the roofline figures)
   if (lane == 0) {
-    red[0][wid] = n_cand;
-    red[1][wid] = n_pair;
+    s_stat[w][0] = n_cand;
+    s_stat[w][1] = n_pair;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long a = 0, b = 0;
-    for (int w = 0; w < BSP / 64; w++) {
-      a += red[0][w];
-      b += red[1][w];
+    unsigned long long a = 0, bb = 0;
+#pragma unroll
+    for (int k = 0; k < NB_WAVES; k++) {
+      a += s_stat[k][0];
+      bb += s_stat[k][1];
     }
-    if (a) atomicAdd(&cnt->candidates, a);
-    if (b) atomicAdd(&cnt->pairs, b);
+    stat_part[2 * blockIdx.x] = a;
+    stat_part[2 * blockIdx.x + 1] = bb;
   }
 }
 
@@ -613,14 +685,22 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
 // ---------------------------------------------------------------------------------------------
 static inline unsigned nb(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-void launch_neighbours(hipStream_t s, unsigned n_init, const GridInfo &g, const int *offsets,
-                       const int *key_sorted, const double4 *nl4, const double2 *pse2,
-                       const double *h, const uint32_t *flags, const Consts &c, int *nbr,
-                       int nl_stride, int cap, int *count, double *T_t, Counters *cnt,
-                       int want_pse) {
-  hipLaunchKernelGGL(k_neighbours, dim3(nb(n_init, BSP)), dim3(BSP), 0, s, n_init, g, offsets,
-                     key_sorted, nl4, pse2, h, flags, c, nbr, nl_stride, cap, count, T_t, cnt,
-                     want_pse);
+int neighbour_blocks(const GridInfo &g) { return (int)((g.num_cell + NB_WAVES - 1) / NB_WAVES); }
+
+void launch_neighbours(hipStream_t s, const GridInfo &g, const int *offsets, const double4 *nl4,
+                       const double2 *pse2, const double *h, const uint32_t *flags,
+                       const Consts &c, int *nbr, int cap, int *count, double *T_t, Counters *cnt,
+                       int want_pse, unsigned long long *stat_part) {
+  const dim3 grid(neighbour_blocks(g)), block(NB_BLOCK);
+  if (!want_pse)
+    hipLaunchKernelGGL(k_cell_neighbours<0>, grid, block, 0, s, g, offsets, nl4, pse2, h, flags,
+                       c, nbr, cap, count, T_t, cnt, stat_part);
+  else if (c.precision == MFG_FP64)
+    hipLaunchKernelGGL(k_cell_neighbours<1>, grid, block, 0, s, g, offsets, nl4, pse2, h, flags,
+                       c, nbr, cap, count, T_t, cnt, stat_part);
+  else
+    hipLaunchKernelGGL(k_cell_neighbours<2>, grid, block, 0, s, g, offsets, nl4, pse2, h, flags,
+                       c, nbr, cap, count, T_t, cnt, stat_part);
 }
 
 void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *rec, const int *nbr,

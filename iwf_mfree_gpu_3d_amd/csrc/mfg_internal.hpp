@@ -172,11 +172,13 @@ void launch_cells_to_tables(hipStream_t s, const GridInfo &g, const int *offsets
 void launch_bbox_reduce(hipStream_t s, const State &st, const Counters *cnt, unsigned n_init,
                         double *partials, int nblocks);
 
-void launch_neighbours(hipStream_t s, unsigned n_init, const GridInfo &g, const int *offsets,
-                       const int *key_sorted, const double4 *nl4, const double2 *pse2,
-                       const double *h, const uint32_t *flags, const Consts &c, int *nbr,
-                       int nl_stride, int cap, int *count, double *T_t, Counters *cnt,
-                       int want_pse);
+// neighbour lists (+ PSE heat) for every cell; stat_part gets 2 totals (candidates, pairs) per
+// workgroup, neighbour_blocks(g) workgroups
+int neighbour_blocks(const GridInfo &g);
+void launch_neighbours(hipStream_t s, const GridInfo &g, const int *offsets, const double4 *nl4,
+                       const double2 *pse2, const double *h, const uint32_t *flags,
+                       const Consts &c, int *nbr, int cap, int *count, double *T_t, Counters *cnt,
+                       int want_pse, unsigned long long *stat_part);
 void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *rec, const int *nbr,
                  int nl_stride, const int *count, const Consts &c, const Counters *cnt);
 

@@ -102,6 +102,7 @@ def _declare(L):
         "o_advance_time": (None, [_vp]),
         "o_run": (C.c_int, [_vp, C.c_int, C.c_int]),
         "o_sort_only": (C.c_int, [_vp]),
+        "o_plasticity_only": (None, [_vp]),
         "o_counts": (None, [_vp, _up, _up]),
         "o_secant_failures": (C.c_long, [_vp]),
         "o_field_width": (C.c_int, [C.c_int]),
@@ -364,6 +365,9 @@ class OracleSim:
 
     def sort_only(self):
         return lib().o_sort_only(self._h)
+
+    def plasticity_only(self):
+        lib().o_plasticity_only(self._h)
 
     def counts(self):
         n, ni = C.c_uint(0), C.c_uint(0)

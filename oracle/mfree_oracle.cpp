@@ -1816,6 +1816,10 @@ int o_sort_only(void *p) {
   return s->status;
 }
 
+// the Johnson-Cook radial return alone on the current state (src/actions_gpu.cu:371-449); for
+// the consistency checks of the return mapping
+void o_plasticity_only(void *p) { plasticity_johnson_cook(*(sim *)p); }
+
 void o_counts(void *p, unsigned *N, unsigned *N_init) {
   sim *s = (sim *)p;
   *N = s->N;

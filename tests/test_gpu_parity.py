@@ -190,7 +190,7 @@ def test_tool_binning_bit_exact(hip_device):
 
 
 def test_thermal_contact_cut_fp64(hip_device):
-    """Single grain, PSE heat with tool particles, hashed contact, friction heat: 120 steps."""
+    """Single grain, PSE heat with tool particles, exhaustive contact, friction heat: 120 steps."""
     sc = scenes.Scene(2, 0.2, 3)
     assert sc.tool_p.sum() > 100
     sim = helpers.oracle_from_scene(sc)
@@ -237,4 +237,37 @@ def test_fp32_against_fp64_oracle(hip_device, precision):
         errs[name] = rel_err(g, o)
     print(f"fp32 (precision {precision}) relative errors vs fp64 oracle", errs)
     for name in ("S", "T", "pos"):
+        assert errs[name] < 1e-4, errs
+
+
+@pytest.mark.parametrize("precision", [_lib.MFG_FP32_KAHAN, _lib.MFG_FP32_WIDE], ids=["kahan", "wide"])
+def test_fp32_thermal_contact_against_fp64_oracle(hip_device, precision):
+    """FP32 modes on the single-grain cut with PSE heat and tool-workpiece thermal contact: after
+    120 steps the stress, position and the temperature rise T - T_init stay within 1e-4 relative
+    L2 of the FP64 oracle (the FP32 PSE terms are accumulated in FP64)."""
+    sc = scenes.Scene(2, 0.2, 3)
+    sim = helpers.oracle_from_scene(sc)
+    dt = sc.params.dt
+    sim.set_time(dt, dt)
+    parts, grid = scenes.setup_cut(sc, precision=precision)
+    api.global_time_current = dt
+    lf = api.leap_frog(sc.n, grid.num_cell())
+    n = 120
+    assert sim.run(n) == 0
+    scenes.run(lf, parts, grid, n)
+    og = np.argsort(parts.download(_lib.F_ID).astype(np.int64))
+    oo = np.argsort(sim.field(O.F_ID).astype(np.int64))
+    T0 = sc.params.trml_wp.T_init
+    errs = {}
+    for f, name in [(O.F_S, "S"), (O.F_POS, "pos"), (O.F_VEL, "vel"), (O.F_T, "T_rise"), (O.F_T_T, "T_t")]:
+        g = parts.download(f)[og]
+        o = sim.field(f)[oo]
+        if f == O.F_S:
+            o = 0.5 * (o + o.reshape(-1, 3, 3).transpose(0, 2, 1).reshape(-1, 9))
+        if f == O.F_T:
+            g, o = g - T0, o - T0
+        errs[name] = rel_err(g, o)
+    print(f"fp32 (precision {precision}) thermal contact relative errors vs fp64 oracle", errs)
+    assert np.abs(sim.field(O.F_T) - T0).max() > 1e-3, "the cut must heat the workpiece"
+    for name in ("S", "pos", "T_rise"):
         assert errs[name] < 1e-4, errs

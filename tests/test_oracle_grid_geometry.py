@@ -218,3 +218,192 @@ def test_step_conserves_linear_momentum():
     p1 = sim.field(O.F_VEL).sum(0)
     np.testing.assert_allclose(p1, p0, rtol=0, atol=1e-12 * np.abs(vel).sum())
     assert np.abs(vt).max() > 0   # the step did produce forces
+
+
+# ---- radial return, leap-frog, contact law, mesh normals ---------------------------------------
+def _material(dx=1e-3):
+    E, nu, rho0 = 1.1, 0.35, 4.43
+    K, G = E / (3 * (1 - 2 * nu)), E / (2 * (1 + nu))
+    phys = O.Phys(E, nu, rho0, K, G, rho0 * dx ** 3)
+    wdp = O.cubic_spline(np.zeros(3), np.array([dx, 0.0, 0.0]), 1.5 * dx, False)[0]
+    corr = O.Corr(wdp, 0.3, 0.5, 1.0, 1.0, 0.1)
+    joco = O.Joco(0.01098, 0.01092, 0.014, 0.93, 1.1, 1878.0, 300.0, 1e-6, 1.0)
+    trml = O.Trml(553e-8, 0.9, 7.1e-13, 7.1e-13 / (4.43 * 553e-8), 300.0, 0.0)
+    return phys, corr, joco, trml
+
+
+def test_radial_return_consistency():
+    """After the return the deviatoric stress lies on the yield surface,
+    |S_new| = sqrt(2/3) sigma_Y(eps_new, eps_dot_new, T) (up to the secant tolerance), it is parallel
+    to the trial stress, eps_pl grows by sqrt(2/3) dlambda and T by tq/(cp rho) d_eps sigma_Y;
+    below the yield surface nothing changes but eps_pl_dot is reset (src/actions_gpu.cu:371-449)."""
+    rng = np.random.default_rng(4)
+    n = 40
+    pos = np.stack([np.arange(n) * 1.0, np.zeros(n), np.zeros(n)], 1)
+    sim = O.OracleSim(pos, rho=np.full(n, 4.43), T=np.linspace(300.0, 900.0, n), h=np.full(n, 1.5e-3))
+    sim.set_options(cspm=False, heat=O.HEAT_NONE, single_sqrt=False)
+    phys, corr, joco, trml = _material()
+    sim.setup_constants(phys, corr, joco, trml)
+    dt = 1e-5
+    sim.set_time(dt, dt)
+    S = np.zeros((n, 3, 3))
+    for i in range(n):
+        a = rng.normal(size=(3, 3))
+        a = a + a.T
+        a -= np.trace(a) / 3 * np.eye(3)
+        S[i] = a / np.linalg.norm(a) * (0.004 if i % 4 == 0 else 0.03)  # every 4th below yield
+    eps0 = rng.uniform(0.05, 0.3, n)
+    epsd0 = rng.uniform(0.0, 1e-2, n)
+    sim.set_field(O.F_S, np.stack([O.glm_from_rowmajor(m) for m in S]))
+    sim.set_field(O.F_EPS_PL, eps0)
+    sim.set_field(O.F_EPS_PL_DOT, epsd0)
+    T0 = sim.field(O.F_T).copy()
+    sim.plasticity_only()
+    Sn = np.stack([O.rowmajor_from_glm(m) for m in sim.field(O.F_S)])
+    eps1, epsd1, T1 = sim.field(O.F_EPS_PL), sim.field(O.F_EPS_PL_DOT), sim.field(O.F_T)
+    mu = phys.G
+    plastic = 0
+    for i in range(n):
+        norm = np.sqrt(np.sum(S[i] * S[i]))
+        sy = O.sigma_yield(joco, eps0[i], epsd0[i], min(T0[i], joco.Tmelt - 1))
+        if np.sqrt(1.5) * norm < sy:
+            np.testing.assert_array_equal(Sn[i], S[i])
+            assert eps1[i] == eps0[i] and epsd1[i] == 0.0 and T1[i] == T0[i]
+            continue
+        plastic += 1
+        dl = (eps1[i] - eps0[i]) / np.sqrt(2 / 3)
+        assert dl > 0
+        assert epsd1[i] == pytest.approx(np.sqrt(2 / 3) * dl / dt, rel=1e-12)
+        np.testing.assert_allclose(Sn[i], S[i] * (1 - 2 * mu * dl / norm), rtol=1e-12, atol=1e-18)
+        sy_new = O.sigma_yield(joco, eps1[i], epsd1[i], T0[i])
+        assert np.sqrt(np.sum(Sn[i] * Sn[i])) == pytest.approx(np.sqrt(2 / 3) * sy_new, rel=1e-4)
+        assert T1[i] - T0[i] == pytest.approx(trml.tq / (trml.cp * 4.43) * (eps1[i] - eps0[i]) * sy_new,
+                                              rel=1e-12)
+    assert plastic >= n // 2
+
+
+def test_leap_frog_free_particles_move_linearly():
+    """Particles farther apart than the kernel support feel no force: the midpoint predictor /
+    corrector then moves them exactly along x0 + t v (src/leap_frog.cu:30-100)."""
+    rng = np.random.default_rng(8)
+    n = 27
+    g = np.stack(np.meshgrid(np.arange(3), np.arange(3), np.arange(3), indexing="ij"), -1).reshape(-1, 3)
+    pos = g * 1e-2 + 5e-2
+    vel = rng.normal(size=(n, 3)) * 1e-4
+    sim = O.OracleSim(pos, vel3=vel, rho=np.full(n, 4.43), T=np.full(n, 300.0), h=np.full(n, 1.5e-3))
+    sim.set_options(cspm=True, heat=O.HEAT_BROOKSHAW)
+    phys, corr, joco, trml = _material()
+    sim.setup_constants(phys, corr, joco, trml)
+    sim.set_grid_locked(np.zeros(3), np.full(3, 0.2), 1.5e-3)
+    dt, steps = 2e-4, 25
+    sim.set_time(dt, dt)
+    assert sim.run(steps) == 0
+    ids = sim.field(O.F_ID).astype(np.int64)
+    np.testing.assert_allclose(sim.field(O.F_POS), (pos + steps * dt * vel)[ids], rtol=0, atol=1e-15)
+    np.testing.assert_array_equal(sim.field(O.F_VEL), vel[ids])
+    assert np.all(sim.field(O.F_S) == 0.0)
+
+
+def _box_mesh(lo, hi):
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    v = np.array([[x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0],
+                  [x0, y0, z1], [x1, y0, z1], [x1, y1, z1], [x0, y1, z1]], float)
+    # counter-clockwise seen from outside: outward face normals
+    t = np.array([[0, 2, 1], [0, 3, 2], [4, 5, 6], [4, 6, 7], [0, 1, 5], [0, 5, 4],
+                  [1, 2, 6], [1, 6, 5], [2, 3, 7], [2, 7, 6], [3, 0, 4], [3, 4, 7]], np.int32)
+    return v, t
+
+
+def test_mesh_normals_of_a_box():
+    """Face normals are the outward axes; angle-weighted vertex normals at a box corner are the
+    normalised sum of its three faces (each face contributes 90 degrees whatever the
+    triangulation); edge normals are the normalised sum of the two adjacent triangles' normals
+    (src/surface_tri_mesh.cpp:410-568)."""
+    v, t = _box_mesh((0.0, 0.0, 0.0), (1.0, 2.0, 0.5))
+    out = O.mesh_normals(v, t)
+    c = v.mean(0)
+    for k in range(t.shape[0]):
+        P = out[k, 0:9].reshape(3, 3)
+        n = out[k, 9:12]
+        np.testing.assert_array_equal(P, v[t[k]])
+        axis = np.zeros(3)
+        j = int(np.argmax(np.abs(n)))
+        axis[j] = np.sign(P.mean(0)[j] - c[j])
+        np.testing.assert_allclose(n, axis, atol=1e-15)
+        for q in range(3):
+            np_q = out[k, 12 + 3 * q:15 + 3 * q]
+            np.testing.assert_allclose(np_q, np.sign(P[q] - c) / np.sqrt(3), atol=1e-15)
+        for q, (a, b) in enumerate(((0, 1), (1, 2), (2, 0))):
+            ne = out[k, 21 + 3 * q:24 + 3 * q]
+            mid = 0.5 * (P[a] + P[b])
+            # an edge of the box: two faces; a face diagonal: one face
+            on = [ax for ax in range(3) if np.isclose(mid[ax], v[:, ax].min()) or np.isclose(mid[ax], v[:, ax].max())]
+            expect = np.zeros(3)
+            for ax in on:
+                expect[ax] = np.sign(mid[ax] - c[ax])
+            expect /= np.linalg.norm(expect)
+            np.testing.assert_allclose(ne, expect, atol=1e-15)
+
+
+def test_contact_force_law_on_a_plane():
+    """One particle a depth d below the top face of a resting box tool: the exhaustive search finds
+    that face, and the penalty force is m d alpha / dt^2 along the outward normal, recorded in
+    fc and in the tool force sum (src/tool_3d_gpu.cu:455-583); no friction with mu = 0."""
+    phys, corr, joco, trml = _material()
+    v, t = _box_mesh((0.0, 0.0, -1.0), (1.0, 1.0, 0.0))
+    d = 2e-4
+    pos = np.array([[0.31, 0.47, -d]])
+    sim = O.OracleSim(pos, rho=np.full(1, 4.43), T=np.full(1, 300.0), h=np.full(1, 1.5e-3))
+    sim.set_options(cspm=True, heat=O.HEAT_NONE)
+    sim.setup_constants(phys, corr, joco, trml)
+    gmin, gmax = np.array([0.25, 0.4, -0.05]), np.array([0.4, 0.55, 0.05])
+    sim.set_grid_locked(gmin, gmax, 1.5e-3)
+    alpha = 0.1
+    sim.add_tool(v, t, vel=np.zeros(3), mass=phys.mass, alpha=alpha, algo=0, mu=0.0)
+    dt = 1e-5
+    sim.set_time(dt, dt)
+    assert sim.step(record=True) == 0
+    fc = sim.field(O.F_FC)[0]
+    expect = phys.mass * d * alpha / dt ** 2
+    np.testing.assert_allclose(fc, [0.0, 0.0, expect], rtol=1e-12, atol=1e-12 * expect)
+    np.testing.assert_allclose(sim.forces()[0], [0.0, 0.0, expect], rtol=1e-6)
+    # a particle outside the box feels nothing
+    sim2 = O.OracleSim(np.array([[0.31, 0.47, +d]]), rho=np.full(1, 4.43), T=np.full(1, 300.0),
+                       h=np.full(1, 1.5e-3))
+    sim2.set_options(cspm=True, heat=O.HEAT_NONE)
+    sim2.setup_constants(phys, corr, joco, trml)
+    sim2.set_grid_locked(gmin, gmax, 1.5e-3)
+    sim2.add_tool(v, t, vel=np.zeros(3), mass=phys.mass, alpha=alpha, algo=0, mu=0.0)
+    sim2.set_time(dt, dt)
+    assert sim2.step(record=True) == 0
+    assert np.all(sim2.field(O.F_FC) == 0.0)
+
+
+def test_boundary_conditions_fixed_particles():
+    """Fixed particles end the step with the prescribed velocity, zero contact force and zero
+    position / velocity rates; with the thermal boundary condition their temperature is reset to
+    the workpiece T_init (src/actions_gpu.cu:451-487)."""
+    dx = 1e-3
+    pos = _lattice(6, 6, 4, dx, 0.0, 1)
+    n = pos.shape[0]
+    rng = np.random.default_rng(6)
+    vel = rng.normal(size=(n, 3)) * 1e-4
+    fixed = (pos[:, 2] < 0.5 * dx).astype(float)     # bottom layer
+    T = 300.0 + rng.uniform(0, 50, n)
+    sim = O.OracleSim(pos, vel3=vel, rho=np.full(n, 4.43), T=T, h=np.full(n, 1.5 * dx), fixed=fixed)
+    sim.set_options(cspm=True, heat=O.HEAT_BROOKSHAW)
+    phys, corr, joco, trml = _material(dx)
+    sim.setup_constants(phys, corr, joco, trml)
+    sim.set_grid_locked(pos.min(0) - 3 * dx, pos.max(0) + 3 * dx, 1.5 * dx)
+    sim.set_time(1e-5, 1e-5)
+    assert sim.step() == 0
+    f = sim.field(O.F_FIXED) == 1.0
+    assert f.sum() == (fixed == 1).sum() > 0
+    np.testing.assert_array_equal(sim.field(O.F_VEL)[f], sim.field(O.F_VEL_BC)[f])
+    assert np.all(sim.field(O.F_FC)[f] == 0.0)
+    assert np.all(sim.field(O.F_POS_T)[f] == 0.0) and np.all(sim.field(O.F_VEL_T)[f] == 0.0)
+    np.testing.assert_array_equal(sim.field(O.F_T)[f], trml.T_init)
+    # free particles moved and kept their own temperature evolution
+    assert np.any(sim.field(O.F_VEL)[~f] != vel[sim.field(O.F_ID).astype(int)][~f])
+    assert np.all(sim.field(O.F_T)[~f] != trml.T_init)
