@@ -20,6 +20,7 @@
 #ifndef MFREE_GPU_3D_HPP_
 #define MFREE_GPU_3D_HPP_
 
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -243,11 +244,24 @@ class tool_forces {
   tool_forces(const tool_forces &) = delete;
   tool_forces &operator=(const tool_forces &) = delete;
   void reset() { mfg_shim::check(mfg_tool_forces_reset(handle)); }
-  // 3 floats per tool, as get_forces_host returns float3 (src/tool_3d_gpu.cu:44-51)
+  // 3 floats per tool, as get_forces_host returns float3 (src/tool_3d_gpu.cu:34-38)
   std::vector<float> get_forces_host() const {
     std::vector<float> f(3 * (size_t)num_tool);
     mfg_shim::check(mfg_tool_forces_get_host(handle, f.data()));
     return f;
+  }
+  // one line of "%.9g %.9g %.9g " per tool to `fp` (src/tool_3d_gpu.cu:53-69 writes
+  // results/forces_new.txt); the reference also prints the forces x 1e7 when verbose
+  void report(FILE *fp, bool verbose = false) const {
+    const std::vector<float> f = get_forces_host();
+    if (verbose) std::printf("Tool Forces:\n");
+    for (size_t k = 0; k < f.size() / 3; k++) {
+      std::fprintf(fp, "%.9g %.9g %.9g ", f[3 * k], f[3 * k + 1], f[3 * k + 2]);
+      if (verbose) std::printf("%f %f %f ", f[3 * k] * 1e7, f[3 * k + 1] * 1e7, f[3 * k + 2] * 1e7);
+    }
+    std::fprintf(fp, "\n");
+    if (verbose) std::printf("\n");
+    std::fflush(fp);
   }
 };
 

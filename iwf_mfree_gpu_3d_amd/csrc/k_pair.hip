@@ -201,12 +201,52 @@ __global__ void __launch_bounds__(NB_BLOCK)
 // neighbour k: no gather waits on its own index, and one gather is always in flight.
 // Rows are 16-byte aligned and hold a multiple of 4 (at least 8) entries; the entries past n
 // that the vector reads pick up are never dereferenced.
-template <class Load, class Body>
-__device__ __forceinline__ void gather_loop(const int *row, int n, Load &&load, Body &&body) {
+#ifndef PAIR_PREFETCH
+#define PAIR_PREFETCH 0
+#endif
+// touch the first word of the records of a group of 4 neighbours (moves their lines towards the
+// caches ahead of the real gathers); the touched values are folded into `sink`
+template <class RecT>
+__device__ __forceinline__ void touch4(const RecT *rec, int a, int b, int c, int d, int n_left,
+                                       unsigned &sink) {
+  const unsigned *base = (const unsigned *)rec;
+  constexpr int W = sizeof(RecT) / 4;
+  unsigned v = base[(size_t)a * W];
+  if (n_left > 1) v ^= base[(size_t)b * W];
+  if (n_left > 2) v ^= base[(size_t)c * W];
+  if (n_left > 3) v ^= base[(size_t)d * W];
+  sink ^= v;
+}
+
+// 1: two record buffers alternate (the gather of k + 1 overlaps the arithmetic of k);
+// 0: each record is gathered where it is used (fewer registers), the touches warm the caches
+#ifndef PAIR_DOUBLE_BUFFER
+#define PAIR_DOUBLE_BUFFER 1
+#endif
+
+template <class RecT, class Load, class Body>
+__device__ __forceinline__ void gather_loop(const int *row, int n, const RecT *rec, Load &&load,
+                                            Body &&body, unsigned &sink) {
   if (n <= 0) return;
   typedef int iv4 __attribute__((ext_vector_type(4)));
+  if (!PAIR_DOUBLE_BUFFER) {
+    iv4 nxt = *(const iv4 *)row;
+    for (int k = 0; k < n; k += 4) {
+      const iv4 cur = nxt;
+      if (k + 4 < n) {
+        nxt = *(const iv4 *)(row + k + 4);
+        if (PAIR_PREFETCH) touch4(rec, nxt.x, nxt.y, nxt.z, nxt.w, n - k - 4, sink);
+      }
+      body(load(cur.x));
+      if (k + 1 < n) body(load(cur.y));
+      if (k + 2 < n) body(load(cur.z));
+      if (k + 3 < n) body(load(cur.w));
+    }
+    return;
+  }
   iv4 cur = *(const iv4 *)row;
   iv4 nxt = n > 4 ? *(const iv4 *)(row + 4) : cur;
+  if (PAIR_PREFETCH && n > 4) touch4(rec, nxt.x, nxt.y, nxt.z, nxt.w, n - 4, sink);
   auto ra = load(cur.x);
   decltype(ra) rb;
   for (int k = 0; k < n; k += 4) {
@@ -220,7 +260,10 @@ __device__ __forceinline__ void gather_loop(const int *row, int n, Load &&load, 
     body(ra);
     if (k + 3 >= n) break;
     cur = nxt;
-    if (k + 8 < n) nxt = *(const iv4 *)(row + k + 8);
+    if (k + 8 < n) {
+      nxt = *(const iv4 *)(row + k + 8);
+      if (PAIR_PREFETCH) touch4(rec, nxt.x, nxt.y, nxt.z, nxt.w, n - k - 8, sink);
+    }
     if (k + 4 < n) ra = load(cur.x);
     body(rb);
   }
@@ -373,6 +416,7 @@ __global__ void __launch_bounds__(BSP, PAIR64_WAVES)
   const double fac = (INV_PI_D)*h1 * h1 * h1;
   const int n = count[i];
   const int *row = nbr + (size_t)i * stride;
+  unsigned sink = 0;  // folded values of the prefetch touches
   const int single = c.single_sqrt;
   const double mass = c.i_mass;
 
@@ -392,7 +436,7 @@ __global__ void __launch_bounds__(BSP, PAIR64_WAVES)
         double4 p, q;  // (x, y, z, h), (T, vol, c, tool)
       };
       auto load = [&](int j) { return G{*(const double4 *)&rec[j].x, *(const double4 *)&rec[j].T}; };
-      gather_loop(row, n, load, [&](const G &g) {
+      gather_loop(row, n, rec, load, [&](const G &g) {
         if (g.q.w == 1.) return;
         const double volj = g.q.y;
         double xij = ri.x - g.p.x, yij = ri.y - g.p.y, zij = ri.z - g.p.z;
@@ -402,7 +446,7 @@ __global__ void __launch_bounds__(BSP, PAIR64_WAVES)
         B.m[0][0] -= volj * xij * wx; B.m[1][0] -= volj * xij * wy; B.m[2][0] -= volj * xij * wz;
         B.m[0][1] -= volj * yij * wx; B.m[1][1] -= volj * yij * wy; B.m[2][1] -= volj * yij * wz;
         B.m[0][2] -= volj * zij * wx; B.m[1][2] -= volj * zij * wy; B.m[2][2] -= volj * zij * wz;
-      });
+      }, sink);
     }
     if (det3(B) > 1e-8) inv3(B, invB);
   }
@@ -490,10 +534,11 @@ __global__ void __launch_bounds__(BSP, PAIR64_WAVES)
   // two record buffers alternate: the gather of neighbour k + 1 is in flight during the
   // arithmetic of neighbour k (the sums still run in neighbour-list order)
 #if PAIR64_PIPELINE
-  gather_loop(row, n, [&](int j) { return rec[j]; }, body);
+  gather_loop(row, n, rec, [&](int j) { return rec[j]; }, body, sink);
 #else
   for (int k = 0; k < n; k++) body(rec[row[k]]);
 #endif
+  if (PAIR_PREFETCH && sink == 0xffffffffu && n < 0) return;  // keeps the touches alive
   particle_update(st, i, fl, c, ps, brook, thermal_alpha);
 }
 
@@ -546,6 +591,7 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
   const float fac = (float)INV_PI_D * h1 * h1 * h1;
   const int n = count[i];
   const int *row = nbr + (size_t)i * stride;
+  unsigned sink = 0;  // folded values of the prefetch touches
   const float mass = (float)c.i_mass;
 
   float iB[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};  // invB[col][row] flattened
@@ -555,7 +601,7 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
     for (int q = 0; q < 9; q++) B[q].zero();
     if (!tool_nz) {
       // x, y, z, (tool, vol) are the first 32 bytes of a record
-      gather_loop(row, n, [&](int j) { return *(const double4 *)&rec[j].x; }, [&](const double4 &g) {
+      gather_loop(row, n, rec, [&](int j) { return *(const double4 *)&rec[j].x; }, [&](const double4 &g) {
         const ToolVol tv = __builtin_bit_cast(ToolVol, g.w);
         if (tv.tool == 1.f) return;
         float xij = (float)(ri.x - g.x), yij = (float)(ri.y - g.y), zij = (float)(ri.z - g.z);
@@ -570,7 +616,7 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
         B[0].add(-xij * gx); B[3].add(-xij * gy); B[6].add(-xij * gz);
         B[1].add(-yij * gx); B[4].add(-yij * gy); B[7].add(-yij * gz);
         B[2].add(-zij * gx); B[5].add(-zij * gy); B[8].add(-zij * gz);
-      });
+      }, sink);
     }
     M3 Bd, inv;
 #pragma unroll
@@ -664,7 +710,7 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
       Tl.add(2.0f * rj.vol * (Ti - rj.T) * rinv * e_w);
     }
   };
-  gather_loop(row, n, [&](int j) { return rec[j]; }, body);
+  gather_loop(row, n, rec, [&](int j) { return rec[j]; }, body, sink);
   PairSums ps;
 #pragma unroll
   for (int a = 0; a < 3; a++)
@@ -679,6 +725,7 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
     ps.xt[a] = xt[a].val();
   }
   ps.Tl = Tl.val();
+  if (PAIR_PREFETCH && sink == 0xffffffffu && n < 0) return;  // keeps the touches alive
   particle_update(st, i, fl, c, ps, brook, thermal_alpha);
 }
 

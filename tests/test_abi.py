@@ -63,3 +63,16 @@ def test_scene_builder_presets():
     s2 = scenes.Scene(2, 0.2, 1)
     assert len(s2.tools) == 1 and 4000 < s2.tools[0][1].shape[0] < 6000
     assert s2.tool_p.sum() > 0
+
+
+def test_cpp_shim_compiles_and_driver_built(tmp_path):
+    """The reference-named C++ classes compile on their own (C++17, warnings as errors) and the
+    reference-shaped host driver was built and linked against the library."""
+    import subprocess
+    src = tmp_path / "use_shim.cpp"
+    src.write_text('#define MFG_DEFINE_GLOBALS\n#include "mfree_gpu_3d.hpp"\n'
+                   "int main() { leap_frog *lf = nullptr; (void)lf; return global_time_step; }\n")
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+                        f"-I{os.path.join(ROOT, 'include')}", str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert os.access(os.path.join(ROOT, "iwf_mfree_gpu_3d_amd", "mfree_gpu_3d"), os.X_OK)
