@@ -249,6 +249,18 @@ int mfg_profile_reset(void);
 int mfg_profile_count(void);
 int mfg_profile_get(int k, char *name48, long *launches, double *total_ms);
 
+/* ---- tool mesh readers (set-up time, host only) ------------------------------------------- */
+/* Surface of an LS-DYNA keyword solid mesh (quads = 0: tetrahedra, 1: hexahedra), as
+ * ldynak_read_triangles_from_tetmesh / _quadmesh (src/ldynak_reader.cpp:308-434), and a MeshLab
+ * JSON triangle mesh, as json_read_triangles (src/json_reader.cpp:108-117).  Call once with NULL
+ * arrays for the sizes, then with arrays of 3 * n_verts doubles and 3 * n_tris ints.  Vertices
+ * come by ascending node id and faces in element order (the reference's order follows
+ * std::unordered_map iteration). */
+int mfg_mesh_read_ldynak(const char *path, int quads, int *n_verts, int *n_tris, double *verts3,
+                         int *tris3);
+int mfg_mesh_read_json(const char *path, int *n_verts, int *n_tris, double *verts3, int *tris3);
+const char *mfg_mesh_last_error(void);
+
 /* ---- workload builder (set-up time, host only) --------------------------------------------- */
 typedef struct {
   int nx, ny, nz;             /* workpiece lattice */

@@ -180,6 +180,9 @@ SIGNATURES = {
     "mfg_scene_get_particles": (_i, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "mfg_scene_tool_size": (_i, [_vp, _i, _ip, _ip]),
     "mfg_scene_tool_mesh": (_i, [_vp, _i, _dp, _ip]),
+    "mfg_mesh_read_ldynak": (_i, [C.c_char_p, _i, _ip, _ip, _dp, _ip]),
+    "mfg_mesh_read_json": (_i, [C.c_char_p, _ip, _ip, _dp, _ip]),
+    "mfg_mesh_last_error": (C.c_char_p, []),
 }
 
 _lib = None

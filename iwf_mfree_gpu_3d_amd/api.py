@@ -509,3 +509,29 @@ def profile_read() -> dict:
 
 def synchronize() -> None:
     check(_L().mfg_synchronize())
+
+
+# ---- tool mesh readers ---------------------------------------------------------------------
+def _read_mesh(call, *args):
+    nv, nt = C.c_int(0), C.c_int(0)
+    rc = call(*args, C.byref(nv), C.byref(nt), None, None)
+    if rc != 0:
+        raise MfgError(rc, _L().mfg_mesh_last_error().decode())
+    verts = np.zeros((nv.value, 3))
+    tris = np.zeros((nt.value, 3), dtype=np.int32)
+    rc = call(*args, C.byref(nv), C.byref(nt), verts.ctypes.data_as(C.POINTER(C.c_double)),
+              tris.ctypes.data_as(C.POINTER(C.c_int)))
+    if rc != 0:
+        raise MfgError(rc, _L().mfg_mesh_last_error().decode())
+    return verts, tris
+
+
+def read_mesh_ldynak(path: str, quads: bool = False):
+    """(verts (n, 3), tris (m, 3)) of the surface of an LS-DYNA keyword tet (or hex) mesh,
+    ldynak_read_triangles_from_tetmesh / _quadmesh (src/ldynak_reader.cpp:308-434)."""
+    return _read_mesh(_L().mfg_mesh_read_ldynak, path.encode(), int(bool(quads)))
+
+
+def read_mesh_json(path: str):
+    """(verts, tris) of a MeshLab JSON triangle mesh, json_read_triangles (src/json_reader.cpp:108-117)."""
+    return _read_mesh(_L().mfg_mesh_read_json, path.encode())

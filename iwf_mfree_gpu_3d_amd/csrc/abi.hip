@@ -11,6 +11,7 @@
 #include <cfloat>
 #include <climits>
 #include <cstdarg>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -41,6 +42,9 @@ struct Lib {
   int device = -1;
   hipStream_t stream = nullptr;
   mfg_options opt{0, MFG_HEAT_BROOKSHAW, 1, 256};
+  // exhaustive contact: closest triangle found by one wavefront per (particle, tool) pair;
+  // MFG_LANE_CONTACT=0 in the environment selects the one-thread-per-particle search instead
+  bool opt_contact_search = true;
   // constants as the two reference translation units hold them
   mfg_phys_constants a_phys{}, i_phys{};
   mfg_corr_constants a_corr{}, i_corr{};
@@ -185,6 +189,13 @@ struct mfg_particles {
   int *flag_d = nullptr;
   unsigned long long *stat_part = nullptr;  // per-workgroup (candidates, pairs) of the last list build
   int stat_cap = 0, stat_n = 0;
+  // lane-parallel exhaustive contact search (lazily sized)
+  int2 *cs_list = nullptr;
+  unsigned *cs_n = nullptr;
+  int *cs_slot = nullptr;
+  double *cs_res = nullptr;
+  unsigned cs_cap = 0;
+  int cs_tools = 0;
   const State &st() const { return buf[cur]; }
 };
 
@@ -267,7 +278,8 @@ size_t padded(size_t n) { return (n + 63) / 64 * 64; }
 void free_particles(mfg_particles *p) {
   void *ptrs[] = {p->arena[0], p->arena[1], p->uarena[0], p->uarena[1], p->rec, p->nl4, p->pse2,
                   p->nbr, p->count, p->key, p->key_sorted, p->order, p->blk, p->cnt, p->partials,
-                  p->p_dbg, p->R_dbg, p->flag_d, p->stat_part};
+                  p->p_dbg, p->R_dbg, p->flag_d, p->stat_part, p->cs_list, p->cs_n,
+                  p->cs_slot, p->cs_res};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (p->cnt_h) (void)hipHostFree(p->cnt_h);
@@ -378,6 +390,31 @@ void permute(mfg_grid *g, mfg_particles *p, const Consts &c, int predict) {
   launch_permute_predict_ex(g_lib.stream, p->buf[p->cur], p->buf[nxt], p->N_init, gb, p->cnt, c,
                             predict, p->rec, p->nl4, p->pse2, p->p_dbg, p->R_dbg);
   p->cur = nxt;
+}
+
+// buffers of the lane-parallel exhaustive contact search for n_tools tools
+int contact_search_buffers(mfg_particles *p, int n_tools, ContactSearch &cs) {
+  const unsigned cap = (unsigned)std::min<size_t>((size_t)p->N_init * (size_t)n_tools, (size_t)1 << 22);
+  if (n_tools > p->cs_tools || cap > p->cs_cap) {
+    void *old[] = {p->cs_list, p->cs_n, p->cs_slot, p->cs_res};
+    for (void *q : old)
+      if (q) (void)hipFree(q);
+    p->cs_list = nullptr; p->cs_n = nullptr; p->cs_slot = nullptr; p->cs_res = nullptr;
+    p->cs_cap = 0; p->cs_tools = 0;
+    if (hipMalloc(&p->cs_list, sizeof(int2) * cap) != hipSuccess ||
+        hipMalloc(&p->cs_n, sizeof(unsigned)) != hipSuccess ||
+        hipMalloc(&p->cs_slot, sizeof(int) * (size_t)p->N_init * n_tools) != hipSuccess ||
+        hipMalloc(&p->cs_res, sizeof(double) * 6 * (size_t)cap) != hipSuccess)
+      return fail(MFG_ERR_NOMEM, "contact search buffers (%u entries)", cap);
+    p->cs_cap = cap;
+    p->cs_tools = n_tools;
+  }
+  cs.list = p->cs_list;
+  cs.list_n = p->cs_n;
+  cs.cap = p->cs_cap;
+  cs.slot = p->cs_slot;
+  cs.res = p->cs_res;
+  return MFG_OK;
 }
 
 // neighbour lists (+ PSE) of the current records on grid g
@@ -557,6 +594,8 @@ int mfg_init(int device) {
   if (device < 0 || device >= n) return fail(MFG_ERR_ARG, "device %d of %d", device, n);
   HIP_TRY(hipSetDevice(device));
   HIP_TRY(hipStreamCreateWithFlags(&g_lib.stream, hipStreamNonBlocking));
+  const char *lane_contact = std::getenv("MFG_LANE_CONTACT");
+  g_lib.opt_contact_search = !(lane_contact && lane_contact[0] == '0');
   g_lib.device = device;
   g_lib.ready = true;
   return MFG_OK;
@@ -1503,6 +1542,8 @@ int mfg_leap_frog_step(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int rec
     v.cells = t->cells;
     v.cand_off = t->algo == MFG_CONTACT_EXHAUSTIVE ? t->cand_off : nullptr;
     v.cand_idx = t->cand_idx;
+    v.slot = nullptr;
+    v.res = nullptr;
     for (int a = 0; a < 3; a++) v.corg[a] = t->gbbmin[a] + t->vel[a] * t->mesh_tc;
     v.nx = t->gnx; v.ny = t->gny; v.nz = t->gnz;
     v.active = (t->n_boxed > 0 && t_current >= t->birth && t_current <= t->death) ? 1 : 0;
@@ -1518,6 +1559,20 @@ int mfg_leap_frog_step(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int rec
   }
   if (n_tools > 0)
     for (int a = 0; a < 3; a++) ts.fric_vel[a] = tools[0]->vel[a];
+  int n_exh = 0;
+  for (int k = 0; k < n_tools; k++) n_exh += (ts.t[k].algo == MFG_CONTACT_EXHAUSTIVE && ts.t[k].active);
+  if (n_exh > 0 && g_lib.opt_contact_search) {
+    ProfScope ps("contact_search");
+    ContactSearch cs;
+    rc = contact_search_buffers(p, n_tools, cs);
+    if (rc != MFG_OK) return rc;
+    for (int k = 0; k < n_tools; k++)
+      if (ts.t[k].algo == MFG_CONTACT_EXHAUSTIVE) {
+        ts.t[k].slot = cs.slot + (size_t)k * p->N_init;
+        ts.t[k].res = cs.res;
+      }
+    launch_contact_search(s, p->st(), p->N_init, ts, p->cnt, cs);
+  }
   {
     ProfScope ps("contact_bc");
     launch_contact_bc(s, p->st(), p->N_init, ts, c, p->cnt, p->partials, p->contact_blocks);

@@ -353,6 +353,101 @@ __device__ __forceinline__ void closest_by_candidates(const ToolView &tv, V3 qp,
   }
 }
 
+// ---- exhaustive contact, lane-parallel search ----------------------------------------------------
+__device__ __forceinline__ bool in_tool_box(const ToolView &tv, V3 qp) {
+  return qp.x > tv.bbmin[0] && qp.x < tv.bbmax[0] && qp.y > tv.bbmin[1] && qp.y < tv.bbmax[1] &&
+         qp.z > tv.bbmin[2] && qp.z < tv.bbmax[2];
+}
+
+__global__ void k_reset_u32(unsigned *p) {
+  if (threadIdx.x == 0) *p = 0u;
+}
+
+// list every eligible particle inside the box of an active exhaustive tool (the same test as
+// k_contact_bc) and remember its entry
+__global__ void __launch_bounds__(BST)
+    k_contact_list(State st, unsigned n_init, ToolSet ts, const Counters *cnt, ContactSearch cs) {
+  const unsigned i = blockIdx.x * BST + threadIdx.x;
+  if (i >= n_init || i >= cnt->N) return;
+  const uint32_t fl = st.flags[i];
+  if ((fl & FL_BLANKED) || (fl & FL_TOOL)) return;
+  const V3 qp = v3(st.x[i], st.y[i], st.z[i]);
+  for (int k = 0; k < ts.n; k++) {
+    const ToolView &tv = ts.t[k];
+    if (!tv.active || tv.algo == MFG_CONTACT_HASHED || !tv.slot) continue;
+    if (!in_tool_box(tv, qp)) continue;
+    const unsigned pos = atomicAdd(cs.list_n, 1u);
+    int sl = -1;
+    if (pos < cs.cap) {
+      cs.list[pos] = make_int2((int)i, k);
+      sl = (int)pos;
+    }
+    cs.slot[(size_t)k * n_init + i] = sl;  // -1: searched per thread by k_contact_bc
+  }
+}
+
+// one wavefront per listed pair: lanes take the candidate triangles (the cell's list, or all
+// triangles outside the grid) in strides of 64; the smallest squared distance wins, the
+// smaller triangle index on ties -- the first minimum of the reference's ascending loop
+__global__ void __launch_bounds__(BST)
+    k_contact_search(State st, ToolSet ts, ContactSearch cs) {
+  const int lane = threadIdx.x & 63;
+  const unsigned nw = gridDim.x * (BST / 64);
+  const unsigned w0 = blockIdx.x * (BST / 64) + (threadIdx.x >> 6);
+  const unsigned n = min(*cs.list_n, cs.cap);
+  for (unsigned e = w0; e < n; e += nw) {
+    const int2 ent = cs.list[e];
+    const int k = __builtin_amdgcn_readfirstlane(ent.y);  // one entry per wavefront
+    const unsigned i = (unsigned)__builtin_amdgcn_readfirstlane(ent.x);
+    const ToolView &tv = ts.t[k];
+    const V3 qp = v3(st.x[i], st.y[i], st.z[i]);
+    const double fx = (qp.x - tv.corg[0]) / tv.dx, fy = (qp.y - tv.corg[1]) / tv.dx,
+                 fz = (qp.z - tv.corg[2]) / tv.dx;
+    const bool in_grid = tv.cand_off && fx >= 0. && fy >= 0. && fz >= 0. && fx < tv.nx &&
+                         fy < tv.ny && fz < tv.nz;
+    int beg = 0, end = tv.n_tris;
+    if (in_grid) {
+      const int cell = ((int)fx * tv.ny + (int)fy) * tv.nz + (int)fz;
+      beg = tv.cand_off[cell];
+      end = tv.cand_off[cell + 1];
+    }
+    double best = FLT_MAX;
+    int best_t = 0x7fffffff;
+    V3 cp = v3(0., 0., 0.), nrm = v3(0., 0., 0.);
+    for (int q = beg + lane; q < end; q += 64) {
+      const int t = in_grid ? tv.cand_idx[q] : q;
+      V3 c2, n2;
+      closest_point(tv.tris[t], qp, c2, n2);
+      const double d2 = dist2(c2, qp);
+      if (d2 < best) {  // a lane's candidates ascend, so its first minimum has its smallest index
+        best = d2;
+        best_t = t;
+        cp = c2;
+        nrm = n2;
+      }
+    }
+    double gb = best;
+    int gt = best_t;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ob = __shfl_xor(gb, o, 64);
+      const int ot = __shfl_xor(gt, o, 64);
+      if (ob < gb || (ob == gb && ot < gt)) {
+        gb = ob;
+        gt = ot;
+      }
+    }
+    double *r = cs.res + 6 * (size_t)e;
+    if (gt == 0x7fffffff) {  // no distance below FLT_MAX: the per-thread loop leaves zeros
+      if (lane == 0)
+        for (int k = 0; k < 6; k++) r[k] = 0.;
+    } else if (best_t == gt && best == gb) {
+      r[0] = cp.x; r[1] = cp.y; r[2] = cp.z;
+      r[3] = nrm.x; r[4] = nrm.y; r[5] = nrm.z;
+    }
+  }
+}
+
 // normal orientation test of the tool_3d constructor (host unhash: unsigned casts)
 __global__ void k_orient(const Tri *boxed, const int *cells, int nx, int ny, int nz, double b0,
                          double b1, double b2, double dx, double q0, double q1, double q2,
@@ -419,9 +514,7 @@ __global__ void __launch_bounds__(BST)
         for (int k = 0; k < ts.n; k++) {
           const ToolView &tv = ts.t[k];
           if (!tv.active) continue;
-          bool in = qp.x > tv.bbmin[0] && qp.x < tv.bbmax[0] && qp.y > tv.bbmin[1] &&
-                    qp.y < tv.bbmax[1] && qp.z > tv.bbmin[2] && qp.z < tv.bbmax[2];
-          if (!in) continue;
+          if (!in_tool_box(tv, qp)) continue;
           V3 cp = v3(0, 0, 0), nrm = v3(0, 0, 0);
           if (tv.algo == MFG_CONTACT_HASHED) {
             int ix = (int)(unsigned)((qp.x - tv.bbmin[0]) / tv.dx);
@@ -440,7 +533,12 @@ __global__ void __launch_bounds__(BST)
                          fz = (qp.z - tv.corg[2]) / tv.dx;
             const bool in_grid = tv.cand_off && fx >= 0. && fy >= 0. && fz >= 0. &&
                                  fx < tv.nx && fy < tv.ny && fz < tv.nz;
-            if (in_grid) {
+            const int sl = tv.slot ? tv.slot[i] : -1;
+            if (sl >= 0) {
+              const double *r = tv.res + 6 * (size_t)sl;
+              cp = v3(r[0], r[1], r[2]);
+              nrm = v3(r[3], r[4], r[5]);
+            } else if (in_grid) {
               const int cell = ((int)fx * tv.ny + (int)fy) * tv.nz + (int)fz;
               closest_by_candidates(tv, qp, cell, cp, nrm);
             } else {
@@ -680,6 +778,13 @@ void launch_tool_update(hipStream_t s, Tri *tris, const double *p_init, int n, d
   if (n > 0)
     hipLaunchKernelGGL(k_tool_update, dim3(nbk(n)), dim3(BST), 0, s, tris, p_init, n, t, vel[0],
                        vel[1], vel[2]);
+}
+
+void launch_contact_search(hipStream_t s, const State &st, unsigned n_init, const ToolSet &ts,
+                           const Counters *cnt, const ContactSearch &cs) {
+  hipLaunchKernelGGL(k_reset_u32, dim3(1), dim3(64), 0, s, cs.list_n);
+  hipLaunchKernelGGL(k_contact_list, dim3(nbk(n_init)), dim3(BST), 0, s, st, n_init, ts, cnt, cs);
+  hipLaunchKernelGGL(k_contact_search, dim3(1024), dim3(BST), 0, s, st, ts, cs);
 }
 
 void launch_contact_bc(hipStream_t s, const State &st, unsigned n_init, const ToolSet &ts,

@@ -124,6 +124,9 @@ struct ToolView {
   const int *cand_off;   // exhaustive: closest-triangle candidate lists per grid cell (CSR)
   const int *cand_idx;   //   (triangle indices, ascending within a cell)
   double corg[3];        // exhaustive: grid origin in the mesh's current position
+  const int *slot;       // exhaustive: per particle, its entry in the search list (or -1); null
+                         //   when the closest triangle is searched per thread
+  const double *res;     // per search-list entry: closest point, normal (6 doubles)
   int n_tris;            // triangles in `tris`
   int algo;              // MFG_CONTACT_*
   int nx, ny, nz;
@@ -184,6 +187,17 @@ void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *re
 
 void launch_tool_update(hipStream_t s, Tri *tris, const double *p_init, int n, double t,
                         const double vel[3]);
+// exhaustive contact, lane-parallel: list the (particle, tool) pairs inside a tool box, then one
+// wavefront per pair finds the closest triangle; k_contact_bc reads the results
+struct ContactSearch {
+  int2 *list;           // (particle, tool)
+  unsigned *list_n;     // entries appended this step
+  unsigned cap;         // list capacity
+  int *slot;            // n_init per tool: entry of the particle, -1 if not listed
+  double *res;          // 6 doubles per entry
+};
+void launch_contact_search(hipStream_t s, const State &st, unsigned n_init, const ToolSet &ts,
+                           const Counters *cnt, const ContactSearch &cs);
 void launch_contact_bc(hipStream_t s, const State &st, unsigned n_init, const ToolSet &ts,
                        const Consts &c, const Counters *cnt, float *partials, int nblocks);
 void launch_force_reduce(hipStream_t s, const float *partials, int nblocks, int n_tools,
