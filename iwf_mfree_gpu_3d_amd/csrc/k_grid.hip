@@ -335,7 +335,7 @@ __global__ void __launch_bounds__(BS)
   double cs = sqrt_site(c.i_K / rhoh, c.single_sqrt);
   store_rec<RecT>(rec, i, xh, h, vh, rhoh, A, R, Th, vol, cs, tool ? 1.0 : 0.0);
   nl4[i] = make_double4(xh[0], xh[1], xh[2], blanked ? 1.0 : 0.0);
-  if (pse2) pse2[i] = make_double2(Th, rhoh);
+  if (pse2) pse2[i] = make_double2(Th, c.i_mass / rhoh);  // (T, m / rho) of the PSE sum
   if (p_dbg) {
     p_dbg[i] = p;
 #pragma unroll

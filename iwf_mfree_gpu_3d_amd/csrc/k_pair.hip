@@ -31,9 +31,9 @@ constexpr int BSP = 256;
 #ifndef PAIR64_WAVES
 #define PAIR64_WAVES 1
 #endif
-// 1: the gather of neighbour k + 1 is issued before the arithmetic of neighbour k (FP64 pass)
-#ifndef PAIR64_PIPELINE
-#define PAIR64_PIPELINE 1
+// 1: the FP64 pass gathers each record where it is used (with PAIR64_WAVES 2)
+#ifndef PAIR64_SINGLE
+#define PAIR64_SINGLE 0
 #endif
 constexpr double PI_D = 3.14159265358979323846;
 constexpr double INV_PI_D = 0.31830988618379067154;
@@ -41,20 +41,31 @@ constexpr double INV_PI_D = 0.31830988618379067154;
 // ---------------------------------------------------------------------------------------------
 // One wavefront per cell.  All particles of a cell share the 25 candidate ranges, so the wave
 // lists the candidates once (in LDS, in the reference's visiting order) and its lanes take
-// candidates: the candidate loads are coalesced and no lane idles on a shorter range.  For each
-// particle of the cell, in sorted order, an ordered ballot compaction appends the kept
-// candidates to the particle's row-major list, and the PSE sum is a per-lane partial reduced
-// over the wave at the end.
+// candidates: the candidate loads are coalesced.  The cell's particles are taken in groups of up
+// to NB_GROUP; each 64-candidate chunk is loaded once and tested against every particle of the
+// group (their data broadcast from LDS), which amortises the loads and gives the wave
+// independent work to overlap.  For each particle an ordered ballot compaction appends the kept
+// candidates to its row-major list; its PSE sum is a per-lane partial reduced over the wave.
 //   PSE_MODE 0: no PSE sum; 1: FP64 PSE terms; 2: FP32 PSE terms accumulated in FP64
 constexpr int NB_WAVES = 4;             // cells (wavefronts) per workgroup
 constexpr int NB_BLOCK = 64 * NB_WAVES;
 constexpr int NB_LCAP = 1024;           // candidates listed in LDS per cell (beyond: searched)
+#ifndef NB_GROUP_SIZE
+#define NB_GROUP_SIZE 4
+#endif
+constexpr int NB_GROUP = NB_GROUP_SIZE;  // particles of a cell tested per candidate load
 
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+
+struct NbParticle {
+  double x, y, z, fourh2;
+  double Ti, coef, inv_h2;
+  int blanked, pad;
+};
 
 template <int PSE_MODE>
 __global__ void __launch_bounds__(NB_BLOCK)
@@ -66,6 +77,7 @@ __global__ void __launch_bounds__(NB_BLOCK)
   __shared__ int s_cand[NB_WAVES][NB_LCAP];
   __shared__ int s_pref[NB_WAVES][32];
   __shared__ int s_rst[NB_WAVES][32];
+  __shared__ NbParticle s_pd[NB_WAVES][NB_GROUP];
   __shared__ unsigned long long s_stat[NB_WAVES][2];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -120,62 +132,91 @@ __global__ void __launch_bounds__(NB_BLOCK)
       }
       return s_rst[w][lo] + (q - s_pref[w][lo]);
     };
-    for (int p = s; p < e; p++) {
-      const double4 pi = nl4[p];
-      const bool blanked_i = pi.w != 0.;
-      const double hi = h[p];
-      const double fourh2 = 4 * hi * hi;
-      double Ti = 0., coef = 0., h2 = 0.;
-      if (PSE_MODE) {
-        Ti = pse2[p].x;
-        h2 = hi * hi;
-        const double h4 = h2 * h2;
-        const double h5 = hi * h4;
-        coef = +4. / (h5 * PI_D * sqrt_rn(PI_D));
+    for (int g0 = s; g0 < e; g0 += NB_GROUP) {
+      const int gn = min(NB_GROUP, e - g0);
+      if (lane < gn) {
+        const int p = g0 + lane;
+        const double4 pi = nl4[p];
+        const double hi = h[p];
+        NbParticle d;
+        d.x = pi.x; d.y = pi.y; d.z = pi.z;
+        d.fourh2 = 4 * hi * hi;
+        d.blanked = pi.w != 0. ? 1 : 0;
+        d.pad = 0;
+        d.Ti = 0.; d.coef = 0.; d.inv_h2 = 0.;
+        if (PSE_MODE) {
+          d.Ti = pse2[p].x;
+          const double h2 = hi * hi;
+          const double h4 = h2 * h2;
+          const double h5 = hi * h4;
+          d.coef = +4. / (h5 * PI_D * sqrt_rn(PI_D));
+          d.inv_h2 = 1. / h2;
+        }
+        s_pd[w][lane] = d;
       }
-      const float coef_f = (float)coef, inv_h2_f = (float)(1. / h2), mass_f = (float)c.i_mass;
-      int np = 0;
-      double acc = 0.;
-      int *out = nbr + (size_t)p * cap;
+      wave_lds_sync();
+      int np[NB_GROUP];
+      double acc[NB_GROUP];
+#pragma unroll
+      for (int u = 0; u < NB_GROUP; u++) {
+        np[u] = 0;
+        acc[u] = 0.;
+      }
       for (int b0 = 0; b0 < T; b0 += 64) {
         const int q = b0 + lane;
         const bool valid = q < T;
-        const int j = valid ? cand_at(q) : p;
+        const int j = valid ? cand_at(q) : g0;
         const double4 pj = nl4[j];
-        const double xij = pi.x - pj.x, yij = pi.y - pj.y, zij = pi.z - pj.z;
-        const double rr2 = xij * xij + yij * yij + zij * zij;
-        if (PSE_MODE == 1 && valid) {
-          // lapl_pse (src/kernels.cuh:71-92) in the order of src/interactions_gpu.cu:124-147
-          const double2 tr = pse2[j];
-          const double xx = sqrt_rn(rr2);
-          const double w2 = coef * exp(-xx * xx / (h2));
-          acc += (tr.x - Ti) * w2 * c.i_mass / tr.y;
-        }
-        if (PSE_MODE == 2 && valid) {
-          const double2 tr = pse2[j];
-          const float w2 = coef_f * __expf(-(float)rr2 * inv_h2_f);
-          acc += (double)((float)(tr.x - Ti) * w2 * mass_f * __builtin_amdgcn_rcpf((float)tr.y));
-        }
-        const bool ok = valid && !blanked_i && pj.w == 0. && !(rr2 >= fourh2 || rr2 < 1e-8);
-        const unsigned long long m = __ballot(ok);
-        const int pos = np + __popcll(m & ((1ull << lane) - 1ull));
-        if (ok && pos < cap) out[pos] = j;
-        np += __popcll(m);
-      }
-      if (PSE_MODE) {
+        double2 tr = make_double2(0., 1.);
+        if (PSE_MODE) tr = pse2[j];
+        const bool cand_ok = valid && pj.w == 0.;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-      }
-      if (lane == 0) {
-        if (np > cap) {
-          atomicOr(&cnt->err_capacity, 1);
-          atomicMax(&cnt->max_count, np);
+        for (int u = 0; u < NB_GROUP; u++) {
+          if (u < gn) {
+            const NbParticle &d = s_pd[w][u];
+            const double xij = d.x - pj.x, yij = d.y - pj.y, zij = d.z - pj.z;
+            const double rr2 = xij * xij + yij * yij + zij * zij;
+            // lapl_pse (src/kernels.cuh:71-92) times (T_j - T_i) m / rho_j
+            // (src/interactions_gpu.cu:124-147); r^2 / h^2 as r^2 times 1 / h^2 and m / rho_j
+            // from the record, each within two ulps of the reference's sqrt-square and divisions
+            if (PSE_MODE == 1 && valid) {
+              const double w2 = d.coef * exp(-rr2 * d.inv_h2);
+              acc[u] += (tr.x - d.Ti) * w2 * tr.y;
+            }
+            if (PSE_MODE == 2 && valid) {
+              const float w2 = (float)d.coef * __expf(-(float)rr2 * (float)d.inv_h2);
+              acc[u] += (double)((float)(tr.x - d.Ti) * w2 * (float)tr.y);
+            }
+            const bool ok = cand_ok && !d.blanked && !(rr2 >= d.fourh2 || rr2 < 1e-8);
+            const unsigned long long m = __ballot(ok);
+            const int pos = np[u] + __popcll(m & ((1ull << lane) - 1ull));
+            if (ok && pos < cap) nbr[(size_t)(g0 + u) * cap + pos] = j;
+            np[u] += __popcll(m);
+          }
         }
-        count[p] = np > cap ? cap : np;
-        if (PSE_MODE) T_t[p] = ((flags[p] & FL_TOOL) ? c.alpha_tool : c.alpha_wp) * acc;
       }
-      n_cand += (unsigned long long)T;
-      n_pair += (unsigned long long)(np > cap ? cap : np);
+#pragma unroll
+      for (int u = 0; u < NB_GROUP; u++) {
+        if (u < gn) {
+          double a = acc[u];
+          if (PSE_MODE) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+          }
+          const int p = g0 + u;
+          if (lane == 0) {
+            if (np[u] > cap) {
+              atomicOr(&cnt->err_capacity, 1);
+              atomicMax(&cnt->max_count, np[u]);
+            }
+            count[p] = np[u] > cap ? cap : np[u];
+            if (PSE_MODE) T_t[p] = ((flags[p] & FL_TOOL) ? c.alpha_tool : c.alpha_wp) * a;
+          }
+          n_cand += (unsigned long long)T;
+          n_pair += (unsigned long long)(np[u] > cap ? cap : np[u]);
+        }
+      }
+      wave_lds_sync();  // the next group overwrites s_pd
     }
   }
   // candidate / pair totals of this workgroup (work accounting of the roofline figures)
@@ -201,52 +242,25 @@ __global__ void __launch_bounds__(NB_BLOCK)
 // neighbour k: no gather waits on its own index, and one gather is always in flight.
 // Rows are 16-byte aligned and hold a multiple of 4 (at least 8) entries; the entries past n
 // that the vector reads pick up are never dereferenced.
-#ifndef PAIR_PREFETCH
-#define PAIR_PREFETCH 0
-#endif
-// touch the first word of the records of a group of 4 neighbours (moves their lines towards the
-// caches ahead of the real gathers); the touched values are folded into `sink`
-template <class RecT>
-__device__ __forceinline__ void touch4(const RecT *rec, int a, int b, int c, int d, int n_left,
-                                       unsigned &sink) {
-  const unsigned *base = (const unsigned *)rec;
-  constexpr int W = sizeof(RecT) / 4;
-  unsigned v = base[(size_t)a * W];
-  if (n_left > 1) v ^= base[(size_t)b * W];
-  if (n_left > 2) v ^= base[(size_t)c * W];
-  if (n_left > 3) v ^= base[(size_t)d * W];
-  sink ^= v;
-}
-
-// 1: two record buffers alternate (the gather of k + 1 overlaps the arithmetic of k);
-// 0: each record is gathered where it is used (fewer registers), the touches warm the caches
-#ifndef PAIR_DOUBLE_BUFFER
-#define PAIR_DOUBLE_BUFFER 1
-#endif
-
-template <class RecT, class Load, class Body>
-__device__ __forceinline__ void gather_loop(const int *row, int n, const RecT *rec, Load &&load,
-                                            Body &&body, unsigned &sink) {
+// SINGLE: each record is gathered where it is used (fewer registers, no overlap by hand)
+template <bool SINGLE = false, class Load, class Body>
+__device__ __forceinline__ void gather_loop(const int *row, int n, Load &&load, Body &&body) {
   if (n <= 0) return;
   typedef int iv4 __attribute__((ext_vector_type(4)));
-  if (!PAIR_DOUBLE_BUFFER) {
-    iv4 nxt = *(const iv4 *)row;
+  if (SINGLE) {
+    iv4 nx = *(const iv4 *)row;
     for (int k = 0; k < n; k += 4) {
-      const iv4 cur = nxt;
-      if (k + 4 < n) {
-        nxt = *(const iv4 *)(row + k + 4);
-        if (PAIR_PREFETCH) touch4(rec, nxt.x, nxt.y, nxt.z, nxt.w, n - k - 4, sink);
-      }
-      body(load(cur.x));
-      if (k + 1 < n) body(load(cur.y));
-      if (k + 2 < n) body(load(cur.z));
-      if (k + 3 < n) body(load(cur.w));
+      const iv4 cu = nx;
+      if (k + 4 < n) nx = *(const iv4 *)(row + k + 4);
+      body(load(cu.x));
+      if (k + 1 < n) body(load(cu.y));
+      if (k + 2 < n) body(load(cu.z));
+      if (k + 3 < n) body(load(cu.w));
     }
     return;
   }
   iv4 cur = *(const iv4 *)row;
   iv4 nxt = n > 4 ? *(const iv4 *)(row + 4) : cur;
-  if (PAIR_PREFETCH && n > 4) touch4(rec, nxt.x, nxt.y, nxt.z, nxt.w, n - 4, sink);
   auto ra = load(cur.x);
   decltype(ra) rb;
   for (int k = 0; k < n; k += 4) {
@@ -260,13 +274,34 @@ __device__ __forceinline__ void gather_loop(const int *row, int n, const RecT *r
     body(ra);
     if (k + 3 >= n) break;
     cur = nxt;
-    if (k + 8 < n) {
-      nxt = *(const iv4 *)(row + k + 8);
-      if (PAIR_PREFETCH) touch4(rec, nxt.x, nxt.y, nxt.z, nxt.w, n - k - 8, sink);
-    }
+    if (k + 8 < n) nxt = *(const iv4 *)(row + k + 8);
     if (k + 4 < n) ra = load(cur.x);
     body(rb);
   }
+}
+
+// Lanes per particle of the pair pass.  Each of the LPP lanes of a particle takes one
+// contiguous, 4-aligned part of the neighbour list; the partial sums are then added across the
+// lanes (lane 0 + 1, then + the other pair).  Fewer particles in flight per wavefront keep the
+// neighbour records of the particles in flight within the L2 of the XCD.
+#ifndef PAIR_LPP
+#define PAIR_LPP 1
+#endif
+
+// part [k0, k1) of a list of n entries for lane `sub` of LPP (every part starts at a multiple of 4)
+template <int LPP>
+__device__ __forceinline__ void list_part(int n, int sub, int &k0, int &k1) {
+  const int q = ((n + 4 * LPP - 1) / (4 * LPP)) * 4;
+  k0 = min(sub * q, n);
+  k1 = min(k0 + q, n);
+}
+
+// sum of v over the LPP lanes of a particle (lanes sub = 0 .. LPP-1 are adjacent)
+template <int LPP>
+__device__ __forceinline__ double lanes_sum(double v) {
+  if (LPP >= 2) v += __shfl_xor(v, 1, 64);
+  if (LPP >= 4) v += __shfl_xor(v, 2, 64);
+  return v;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -398,10 +433,13 @@ __device__ __forceinline__ void particle_update(const State &st, unsigned i, uin
 
 // ---------------------------------------------------------------------------------------------
 // FP64 pair pass
+template <int LPP>
 __global__ void __launch_bounds__(BSP, PAIR64_WAVES)
     k_pair64(unsigned n_init, State st, const Rec64 *__restrict__ rec, const int *__restrict__ nbr,
              int stride, const int *__restrict__ count, Consts c, const Counters *cnt) {
-  const unsigned i = xcd_block() * BSP + threadIdx.x;
+  const unsigned gt = xcd_block() * BSP + threadIdx.x;
+  const unsigned i = gt / LPP;
+  const int sub = (int)(gt % LPP);
   const unsigned N = cnt->N;
   if (i >= N || i >= n_init) return;
   const uint32_t fl = st.flags[i];
@@ -415,8 +453,10 @@ __global__ void __launch_bounds__(BSP, PAIR64_WAVES)
   const double h1 = 1 / hi;
   const double fac = (INV_PI_D)*h1 * h1 * h1;
   const int n = count[i];
-  const int *row = nbr + (size_t)i * stride;
-  unsigned sink = 0;  // folded values of the prefetch touches
+  int k0, k1;
+  list_part<LPP>(n, sub, k0, k1);
+  const int *row = nbr + (size_t)i * stride + k0;
+  const int nl = k1 - k0;
   const int single = c.single_sqrt;
   const double mass = c.i_mass;
 
@@ -436,7 +476,7 @@ __global__ void __launch_bounds__(BSP, PAIR64_WAVES)
         double4 p, q;  // (x, y, z, h), (T, vol, c, tool)
       };
       auto load = [&](int j) { return G{*(const double4 *)&rec[j].x, *(const double4 *)&rec[j].T}; };
-      gather_loop(row, n, rec, load, [&](const G &g) {
+      gather_loop(row, nl, load, [&](const G &g) {
         if (g.q.w == 1.) return;
         const double volj = g.q.y;
         double xij = ri.x - g.p.x, yij = ri.y - g.p.y, zij = ri.z - g.p.z;
@@ -446,8 +486,12 @@ __global__ void __launch_bounds__(BSP, PAIR64_WAVES)
         B.m[0][0] -= volj * xij * wx; B.m[1][0] -= volj * xij * wy; B.m[2][0] -= volj * xij * wz;
         B.m[0][1] -= volj * yij * wx; B.m[1][1] -= volj * yij * wy; B.m[2][1] -= volj * yij * wz;
         B.m[0][2] -= volj * zij * wx; B.m[1][2] -= volj * zij * wy; B.m[2][2] -= volj * zij * wz;
-      }, sink);
+      });
     }
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) B.m[a][b] = lanes_sum<LPP>(B.m[a][b]);
     if (det3(B) > 1e-8) inv3(B, invB);
   }
 
@@ -533,12 +577,19 @@ __global__ void __launch_bounds__(BSP, PAIR64_WAVES)
   };
   // two record buffers alternate: the gather of neighbour k + 1 is in flight during the
   // arithmetic of neighbour k (the sums still run in neighbour-list order)
-#if PAIR64_PIPELINE
-  gather_loop(row, n, rec, [&](int j) { return rec[j]; }, body, sink);
-#else
-  for (int k = 0; k < n; k++) body(rec[row[k]]);
-#endif
-  if (PAIR_PREFETCH && sink == 0xffffffffu && n < 0) return;  // keeps the touches alive
+  gather_loop<PAIR64_SINGLE != 0>(row, nl, [&](int j) { return rec[j]; }, body);
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      ps.vd.m[a][b] = lanes_sum<LPP>(ps.vd.m[a][b]);
+      ps.sd.m[a][b] = lanes_sum<LPP>(ps.sd.m[a][b]);
+    }
+    ps.vt[a] = lanes_sum<LPP>(ps.vt[a]);
+    ps.xt[a] = lanes_sum<LPP>(ps.xt[a]);
+  }
+  ps.Tl = lanes_sum<LPP>(ps.Tl);
+  if (sub != 0) return;
   particle_update(st, i, fl, c, ps, brook, thermal_alpha);
 }
 
@@ -571,12 +622,14 @@ struct ToolVol {
   float tool, vol;
 };
 
-template <class Acc>
+template <class Acc, int LPP>
 __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
     k_pair32(unsigned n_init, State st, const Rec32 *__restrict__ rec, const int *__restrict__ nbr,
              int stride, const int *__restrict__ count, Consts c, const Counters *cnt) {
 #pragma clang fp contract(fast)
-  const unsigned i = xcd_block() * BSP + threadIdx.x;
+  const unsigned gt = xcd_block() * BSP + threadIdx.x;
+  const unsigned i = gt / LPP;
+  const int sub = (int)(gt % LPP);
   const unsigned N = cnt->N;
   if (i >= N || i >= n_init) return;
   const uint32_t fl = st.flags[i];
@@ -590,8 +643,10 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
   const float h1 = __builtin_amdgcn_rcpf(hi);
   const float fac = (float)INV_PI_D * h1 * h1 * h1;
   const int n = count[i];
-  const int *row = nbr + (size_t)i * stride;
-  unsigned sink = 0;  // folded values of the prefetch touches
+  int k0, k1;
+  list_part<LPP>(n, sub, k0, k1);
+  const int *row = nbr + (size_t)i * stride + k0;
+  const int nl = k1 - k0;
   const float mass = (float)c.i_mass;
 
   float iB[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};  // invB[col][row] flattened
@@ -601,7 +656,7 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
     for (int q = 0; q < 9; q++) B[q].zero();
     if (!tool_nz) {
       // x, y, z, (tool, vol) are the first 32 bytes of a record
-      gather_loop(row, n, rec, [&](int j) { return *(const double4 *)&rec[j].x; }, [&](const double4 &g) {
+      gather_loop(row, nl, [&](int j) { return *(const double4 *)&rec[j].x; }, [&](const double4 &g) {
         const ToolVol tv = __builtin_bit_cast(ToolVol, g.w);
         if (tv.tool == 1.f) return;
         float xij = (float)(ri.x - g.x), yij = (float)(ri.y - g.y), zij = (float)(ri.z - g.z);
@@ -616,13 +671,13 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
         B[0].add(-xij * gx); B[3].add(-xij * gy); B[6].add(-xij * gz);
         B[1].add(-yij * gx); B[4].add(-yij * gy); B[7].add(-yij * gz);
         B[2].add(-zij * gx); B[5].add(-zij * gy); B[8].add(-zij * gz);
-      }, sink);
+      });
     }
     M3 Bd, inv;
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
-      for (int b = 0; b < 3; b++) Bd.m[a][b] = B[3 * a + b].val();
+      for (int b = 0; b < 3; b++) Bd.m[a][b] = lanes_sum<LPP>(B[3 * a + b].val());
     if (det3(Bd) > 1e-8) {
       inv3(Bd, inv);
 #pragma unroll
@@ -710,22 +765,22 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
       Tl.add(2.0f * rj.vol * (Ti - rj.T) * rinv * e_w);
     }
   };
-  gather_loop(row, n, rec, [&](int j) { return rec[j]; }, body, sink);
+  gather_loop(row, nl, [&](int j) { return rec[j]; }, body);
   PairSums ps;
 #pragma unroll
   for (int a = 0; a < 3; a++)
 #pragma unroll
     for (int b = 0; b < 3; b++) {
-      ps.vd.m[a][b] = vd[3 * a + b].val();
-      ps.sd.m[a][b] = sd[3 * a + b].val();
+      ps.vd.m[a][b] = lanes_sum<LPP>(vd[3 * a + b].val());
+      ps.sd.m[a][b] = lanes_sum<LPP>(sd[3 * a + b].val());
     }
 #pragma unroll
   for (int a = 0; a < 3; a++) {
-    ps.vt[a] = vt[a].val();
-    ps.xt[a] = xt[a].val();
+    ps.vt[a] = lanes_sum<LPP>(vt[a].val());
+    ps.xt[a] = lanes_sum<LPP>(xt[a].val());
   }
-  ps.Tl = Tl.val();
-  if (PAIR_PREFETCH && sink == 0xffffffffu && n < 0) return;  // keeps the touches alive
+  ps.Tl = lanes_sum<LPP>(Tl.val());
+  if (sub != 0) return;
   particle_update(st, i, fl, c, ps, brook, thermal_alpha);
 }
 
@@ -752,14 +807,15 @@ void launch_neighbours(hipStream_t s, const GridInfo &g, const int *offsets, con
 
 void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *rec, const int *nbr,
                  int nl_stride, const int *count, const Consts &c, const Counters *cnt) {
+  const dim3 grid(nb((long)n_init * PAIR_LPP, BSP)), block(BSP);
   if (c.precision == MFG_FP64)
-    hipLaunchKernelGGL(k_pair64, dim3(nb(n_init, BSP)), dim3(BSP), 0, s, n_init, st,
-                       (const Rec64 *)rec, nbr, nl_stride, count, c, cnt);
+    hipLaunchKernelGGL((k_pair64<PAIR_LPP>), grid, block, 0, s, n_init, st, (const Rec64 *)rec,
+                       nbr, nl_stride, count, c, cnt);
   else if (c.precision == MFG_FP32_KAHAN)
-    hipLaunchKernelGGL(k_pair32<AccKahan>, dim3(nb(n_init, BSP)), dim3(BSP), 0, s, n_init, st,
+    hipLaunchKernelGGL((k_pair32<AccKahan, PAIR_LPP>), grid, block, 0, s, n_init, st,
                        (const Rec32 *)rec, nbr, nl_stride, count, c, cnt);
   else
-    hipLaunchKernelGGL(k_pair32<AccWide>, dim3(nb(n_init, BSP)), dim3(BSP), 0, s, n_init, st,
+    hipLaunchKernelGGL((k_pair32<AccWide, PAIR_LPP>), grid, block, 0, s, n_init, st,
                        (const Rec32 *)rec, nbr, nl_stride, count, c, cnt);
 }
 
