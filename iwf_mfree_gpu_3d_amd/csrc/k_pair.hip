@@ -35,6 +35,13 @@ constexpr int BSP = 256;
 #ifndef PAIR64_SINGLE
 #define PAIR64_SINGLE 0
 #endif
+// 1: the main loop gathers two neighbours ahead instead of one
+#ifndef PAIR64_DEEP
+#define PAIR64_DEEP 0
+#endif
+#ifndef PAIR32_DEEP
+#define PAIR32_DEEP 0
+#endif
 constexpr double PI_D = 3.14159265358979323846;
 constexpr double INV_PI_D = 0.31830988618379067154;
 
@@ -47,6 +54,8 @@ constexpr double INV_PI_D = 0.31830988618379067154;
 // independent work to overlap.  For each particle an ordered ballot compaction appends the kept
 // candidates to its row-major list; its PSE sum is a per-lane partial reduced over the wave.
 //   PSE_MODE 0: no PSE sum; 1: FP64 PSE terms; 2: FP32 PSE terms accumulated in FP64
+//   F32: the support test starts from an FP32 r^2 (FP32 pair modes); the lists stay those of
+//        the FP64 r^2 of the reference
 constexpr int NB_WAVES = 4;             // cells (wavefronts) per workgroup
 constexpr int NB_BLOCK = 64 * NB_WAVES;
 constexpr int NB_LCAP = 1024;           // candidates listed in LDS per cell (beyond: searched)
@@ -67,7 +76,7 @@ struct NbParticle {
   int blanked, pad;
 };
 
-template <int PSE_MODE>
+template <int PSE_MODE, bool F32>
 __global__ void __launch_bounds__(NB_BLOCK)
     k_cell_neighbours(GridInfo g, const int *__restrict__ off, const double4 *__restrict__ nl4,
                       const double2 *__restrict__ pse2, const double *__restrict__ h,
@@ -175,19 +184,36 @@ __global__ void __launch_bounds__(NB_BLOCK)
           if (u < gn) {
             const NbParticle &d = s_pd[w][u];
             const double xij = d.x - pj.x, yij = d.y - pj.y, zij = d.z - pj.z;
-            const double rr2 = xij * xij + yij * yij + zij * zij;
-            // lapl_pse (src/kernels.cuh:71-92) times (T_j - T_i) m / rho_j
-            // (src/interactions_gpu.cu:124-147); r^2 / h^2 as r^2 times 1 / h^2 and m / rho_j
-            // from the record, each within two ulps of the reference's sqrt-square and divisions
-            if (PSE_MODE == 1 && valid) {
-              const double w2 = d.coef * exp(-rr2 * d.inv_h2);
-              acc[u] += (tr.x - d.Ti) * w2 * tr.y;
+            bool inside;  // 1e-8 <= r^2 < 4 h^2, decided on the FP64 r^2 of the reference
+            if (F32) {
+              // r^2 from FP32 squares of the FP64 differences (relative error <= 4e-7) decides
+              // unless it lies within 4e-6 of a threshold; there the exact FP64 r^2 decides
+              const float xf = (float)xij, yf = (float)yij, zf = (float)zij;
+              const float r2f = xf * xf + yf * yf + zf * zf;
+              const float lim = (float)d.fourh2;
+              const bool near = fabsf(r2f - lim) <= 4e-6f * lim || (r2f < 2e-8f && r2f > 5e-9f);
+              if (near) {
+                const double rr2 = xij * xij + yij * yij + zij * zij;
+                inside = !(rr2 >= d.fourh2 || rr2 < 1e-8);
+              } else {
+                inside = r2f < lim && r2f >= 2e-8f;
+              }
+              if (PSE_MODE == 2 && valid) {
+                const float w2 = (float)d.coef * __expf(-r2f * (float)d.inv_h2);
+                acc[u] += (double)((float)(tr.x - d.Ti) * w2 * (float)tr.y);
+              }
+            } else {
+              const double rr2 = xij * xij + yij * yij + zij * zij;
+              inside = !(rr2 >= d.fourh2 || rr2 < 1e-8);
+              // lapl_pse (src/kernels.cuh:71-92) times (T_j - T_i) m / rho_j
+              // (src/interactions_gpu.cu:124-147); r^2 / h^2 as r^2 times 1 / h^2 and m / rho_j
+              // from the record, each within two ulps of the reference's sqrt-square and divisions
+              if (PSE_MODE == 1 && valid) {
+                const double w2 = d.coef * exp(-rr2 * d.inv_h2);
+                acc[u] += (tr.x - d.Ti) * w2 * tr.y;
+              }
             }
-            if (PSE_MODE == 2 && valid) {
-              const float w2 = (float)d.coef * __expf(-(float)rr2 * (float)d.inv_h2);
-              acc[u] += (double)((float)(tr.x - d.Ti) * w2 * (float)tr.y);
-            }
-            const bool ok = cand_ok && !d.blanked && !(rr2 >= d.fourh2 || rr2 < 1e-8);
+            const bool ok = cand_ok && !d.blanked && inside;
             const unsigned long long m = __ballot(ok);
             const int pos = np[u] + __popcll(m & ((1ull << lane) - 1ull));
             if (ok && pos < cap) nbr[(size_t)(g0 + u) * cap + pos] = j;
@@ -242,11 +268,35 @@ __global__ void __launch_bounds__(NB_BLOCK)
 // neighbour k: no gather waits on its own index, and one gather is always in flight.
 // Rows are 16-byte aligned and hold a multiple of 4 (at least 8) entries; the entries past n
 // that the vector reads pick up are never dereferenced.
-// SINGLE: each record is gathered where it is used (fewer registers, no overlap by hand)
-template <bool SINGLE = false, class Load, class Body>
+// SINGLE: each record is gathered where it is used (fewer registers, no overlap by hand);
+// DEEP: the gathers run two neighbours ahead (three record buffers in flight)
+template <bool SINGLE = false, bool DEEP = false, class Load, class Body>
 __device__ __forceinline__ void gather_loop(const int *row, int n, Load &&load, Body &&body) {
   if (n <= 0) return;
   typedef int iv4 __attribute__((ext_vector_type(4)));
+  if (DEEP) {
+    iv4 cur = *(const iv4 *)row;
+    iv4 nxt = n > 4 ? *(const iv4 *)(row + 4) : cur;
+    auto b0 = load(cur.x);
+    decltype(b0) b1, b2, b3;
+    if (n > 1) b1 = load(cur.y);
+    for (int k = 0; k < n; k += 4) {
+      if (k + 2 < n) b2 = load(cur.z);
+      body(b0);
+      if (k + 1 >= n) break;
+      if (k + 3 < n) b3 = load(cur.w);
+      body(b1);
+      if (k + 2 >= n) break;
+      cur = nxt;
+      if (k + 8 < n) nxt = *(const iv4 *)(row + k + 8);
+      if (k + 4 < n) b0 = load(cur.x);
+      body(b2);
+      if (k + 3 >= n) break;
+      if (k + 5 < n) b1 = load(cur.y);
+      body(b3);
+    }
+    return;
+  }
   if (SINGLE) {
     iv4 nx = *(const iv4 *)row;
     for (int k = 0; k < n; k += 4) {
@@ -577,7 +627,7 @@ __global__ void __launch_bounds__(BSP, PAIR64_WAVES)
   };
   // two record buffers alternate: the gather of neighbour k + 1 is in flight during the
   // arithmetic of neighbour k (the sums still run in neighbour-list order)
-  gather_loop<PAIR64_SINGLE != 0>(row, nl, [&](int j) { return rec[j]; }, body);
+  gather_loop<PAIR64_SINGLE != 0, PAIR64_DEEP != 0>(row, nl, [&](int j) { return rec[j]; }, body);
 #pragma unroll
   for (int a = 0; a < 3; a++) {
 #pragma unroll
@@ -765,7 +815,7 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
       Tl.add(2.0f * rj.vol * (Ti - rj.T) * rinv * e_w);
     }
   };
-  gather_loop(row, nl, [&](int j) { return rec[j]; }, body);
+  gather_loop<false, PAIR32_DEEP != 0>(row, nl, [&](int j) { return rec[j]; }, body);
   PairSums ps;
 #pragma unroll
   for (int a = 0; a < 3; a++)
@@ -794,15 +844,19 @@ void launch_neighbours(hipStream_t s, const GridInfo &g, const int *offsets, con
                        const Consts &c, int *nbr, int cap, int *count, double *T_t, Counters *cnt,
                        int want_pse, unsigned long long *stat_part) {
   const dim3 grid(neighbour_blocks(g)), block(NB_BLOCK);
-  if (!want_pse)
-    hipLaunchKernelGGL(k_cell_neighbours<0>, grid, block, 0, s, g, offsets, nl4, pse2, h, flags,
-                       c, nbr, cap, count, T_t, cnt, stat_part);
-  else if (c.precision == MFG_FP64)
-    hipLaunchKernelGGL(k_cell_neighbours<1>, grid, block, 0, s, g, offsets, nl4, pse2, h, flags,
-                       c, nbr, cap, count, T_t, cnt, stat_part);
-  else
-    hipLaunchKernelGGL(k_cell_neighbours<2>, grid, block, 0, s, g, offsets, nl4, pse2, h, flags,
-                       c, nbr, cap, count, T_t, cnt, stat_part);
+  const bool f32 = c.precision != MFG_FP64;
+#define MFG_NB_LAUNCH(P, F)                                                                     \
+  hipLaunchKernelGGL((k_cell_neighbours<P, F>), grid, block, 0, s, g, offsets, nl4, pse2, h, flags, \
+                     c, nbr, cap, count, T_t, cnt, stat_part)
+  if (!want_pse) {
+    if (f32) MFG_NB_LAUNCH(0, true);
+    else MFG_NB_LAUNCH(0, false);
+  } else if (f32) {
+    MFG_NB_LAUNCH(2, true);
+  } else {
+    MFG_NB_LAUNCH(1, false);
+  }
+#undef MFG_NB_LAUNCH
 }
 
 void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *rec, const int *nbr,

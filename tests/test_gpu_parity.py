@@ -54,9 +54,12 @@ def compare_fields(parts, sim, tol, fields=FIELDS):
     return errs
 
 
-def test_sort_and_cells_bit_exact(hip_device):
+@pytest.mark.parametrize("precision", [_lib.MFG_FP64, _lib.MFG_FP32_KAHAN], ids=["fp64", "fp32"])
+def test_sort_and_cells_bit_exact(hip_device, precision):
     """Hashes, stable sort permutation, Green cell tables, permuted state and neighbour lists are
-    bit-exact, with 10% of the particles hard-blanked (INT_MAX keys at the tail, N < N_init)."""
+    bit-exact, with 10% of the particles hard-blanked (INT_MAX keys at the tail, N < N_init); the
+    FP32 modes decide the support test from an FP32 r^2 with an exact FP64 fallback near the
+    thresholds, and their lists must be the same."""
     sc = perturbed_scene(1, 0.6, 5)
     rng = np.random.default_rng(0)
     blank = rng.random(sc.n) < 0.1
@@ -67,7 +70,7 @@ def test_sort_and_cells_bit_exact(hip_device):
     sim.set_field(O.F_BLANKED, bl)
     sim.set_field(O.F_T_T, tt)
     assert sim.sort_only() == 0
-    parts, grid = scenes.setup_cut(sc)
+    parts, grid = scenes.setup_cut(sc, precision=precision)
     parts.upload(_lib.F_BLANKED, bl)
     parts.upload(_lib.F_T_T, tt)
     grid.assign_hashes(parts)
