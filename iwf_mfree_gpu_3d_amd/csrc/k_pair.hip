@@ -54,8 +54,6 @@ constexpr double INV_PI_D = 0.31830988618379067154;
 // independent work to overlap.  For each particle an ordered ballot compaction appends the kept
 // candidates to its row-major list; its PSE sum is a per-lane partial reduced over the wave.
 //   PSE_MODE 0: no PSE sum; 1: FP64 PSE terms; 2: FP32 PSE terms accumulated in FP64
-//   F32: the support test starts from an FP32 r^2 (FP32 pair modes); the lists stay those of
-//        the FP64 r^2 of the reference
 constexpr int NB_WAVES = 4;             // cells (wavefronts) per workgroup
 constexpr int NB_BLOCK = 64 * NB_WAVES;
 constexpr int NB_LCAP = 1024;           // candidates listed in LDS per cell (beyond: searched)
@@ -76,7 +74,7 @@ struct NbParticle {
   int blanked, pad;
 };
 
-template <int PSE_MODE, bool F32>
+template <int PSE_MODE>
 __global__ void __launch_bounds__(NB_BLOCK)
     k_cell_neighbours(GridInfo g, const int *__restrict__ off, const double4 *__restrict__ nl4,
                       const double2 *__restrict__ pse2, const double *__restrict__ h,
@@ -184,34 +182,18 @@ __global__ void __launch_bounds__(NB_BLOCK)
           if (u < gn) {
             const NbParticle &d = s_pd[w][u];
             const double xij = d.x - pj.x, yij = d.y - pj.y, zij = d.z - pj.z;
-            bool inside;  // 1e-8 <= r^2 < 4 h^2, decided on the FP64 r^2 of the reference
-            if (F32) {
-              // r^2 from FP32 squares of the FP64 differences (relative error <= 4e-7) decides
-              // unless it lies within 4e-6 of a threshold; there the exact FP64 r^2 decides
-              const float xf = (float)xij, yf = (float)yij, zf = (float)zij;
-              const float r2f = xf * xf + yf * yf + zf * zf;
-              const float lim = (float)d.fourh2;
-              const bool near = fabsf(r2f - lim) <= 4e-6f * lim || (r2f < 2e-8f && r2f > 5e-9f);
-              if (near) {
-                const double rr2 = xij * xij + yij * yij + zij * zij;
-                inside = !(rr2 >= d.fourh2 || rr2 < 1e-8);
-              } else {
-                inside = r2f < lim && r2f >= 2e-8f;
-              }
-              if (PSE_MODE == 2 && valid) {
-                const float w2 = (float)d.coef * __expf(-r2f * (float)d.inv_h2);
-                acc[u] += (double)((float)(tr.x - d.Ti) * w2 * (float)tr.y);
-              }
-            } else {
-              const double rr2 = xij * xij + yij * yij + zij * zij;
-              inside = !(rr2 >= d.fourh2 || rr2 < 1e-8);
-              // lapl_pse (src/kernels.cuh:71-92) times (T_j - T_i) m / rho_j
-              // (src/interactions_gpu.cu:124-147); r^2 / h^2 as r^2 times 1 / h^2 and m / rho_j
-              // from the record, each within two ulps of the reference's sqrt-square and divisions
-              if (PSE_MODE == 1 && valid) {
-                const double w2 = d.coef * exp(-rr2 * d.inv_h2);
-                acc[u] += (tr.x - d.Ti) * w2 * tr.y;
-              }
+            const double rr2 = xij * xij + yij * yij + zij * zij;
+            const bool inside = !(rr2 >= d.fourh2 || rr2 < 1e-8);  // src/kernels.cuh:41
+            // lapl_pse (src/kernels.cuh:71-92) times (T_j - T_i) m / rho_j
+            // (src/interactions_gpu.cu:124-147); r^2 / h^2 as r^2 times 1 / h^2 and m / rho_j
+            // from the record, each within two ulps of the reference's sqrt-square and divisions
+            if (PSE_MODE == 1 && valid) {
+              const double w2 = d.coef * exp(-rr2 * d.inv_h2);
+              acc[u] += (tr.x - d.Ti) * w2 * tr.y;
+            }
+            if (PSE_MODE == 2 && valid) {
+              const float w2 = (float)d.coef * __expf(-(float)rr2 * (float)d.inv_h2);
+              acc[u] += (double)((float)(tr.x - d.Ti) * w2 * (float)tr.y);
             }
             const bool ok = cand_ok && !d.blanked && inside;
             const unsigned long long m = __ballot(ok);
@@ -844,18 +826,12 @@ void launch_neighbours(hipStream_t s, const GridInfo &g, const int *offsets, con
                        const Consts &c, int *nbr, int cap, int *count, double *T_t, Counters *cnt,
                        int want_pse, unsigned long long *stat_part) {
   const dim3 grid(neighbour_blocks(g)), block(NB_BLOCK);
-  const bool f32 = c.precision != MFG_FP64;
-#define MFG_NB_LAUNCH(P, F)                                                                     \
-  hipLaunchKernelGGL((k_cell_neighbours<P, F>), grid, block, 0, s, g, offsets, nl4, pse2, h, flags, \
-                     c, nbr, cap, count, T_t, cnt, stat_part)
-  if (!want_pse) {
-    if (f32) MFG_NB_LAUNCH(0, true);
-    else MFG_NB_LAUNCH(0, false);
-  } else if (f32) {
-    MFG_NB_LAUNCH(2, true);
-  } else {
-    MFG_NB_LAUNCH(1, false);
-  }
+#define MFG_NB_LAUNCH(P)                                                                        \
+  hipLaunchKernelGGL((k_cell_neighbours<P>), grid, block, 0, s, g, offsets, nl4, pse2, h, flags, c, \
+                     nbr, cap, count, T_t, cnt, stat_part)
+  if (!want_pse) MFG_NB_LAUNCH(0);
+  else if (c.precision == MFG_FP64) MFG_NB_LAUNCH(1);
+  else MFG_NB_LAUNCH(2);
 #undef MFG_NB_LAUNCH
 }
 

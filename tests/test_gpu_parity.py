@@ -57,9 +57,8 @@ def compare_fields(parts, sim, tol, fields=FIELDS):
 @pytest.mark.parametrize("precision", [_lib.MFG_FP64, _lib.MFG_FP32_KAHAN], ids=["fp64", "fp32"])
 def test_sort_and_cells_bit_exact(hip_device, precision):
     """Hashes, stable sort permutation, Green cell tables, permuted state and neighbour lists are
-    bit-exact, with 10% of the particles hard-blanked (INT_MAX keys at the tail, N < N_init); the
-    FP32 modes decide the support test from an FP32 r^2 with an exact FP64 fallback near the
-    thresholds, and their lists must be the same."""
+    bit-exact, with 10% of the particles hard-blanked (INT_MAX keys at the tail, N < N_init), for
+    particle sets of both precisions."""
     sc = perturbed_scene(1, 0.6, 5)
     rng = np.random.default_rng(0)
     blank = rng.random(sc.n) < 0.1
