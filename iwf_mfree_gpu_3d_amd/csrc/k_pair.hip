@@ -336,6 +336,48 @@ __device__ __forceinline__ double lanes_sum(double v) {
   return v;
 }
 
+// Cooperative gather.  The 64 lanes of a wavefront each need one neighbour record per step.
+// Loaded lane by lane, every 16-byte piece of every record is a separate L1 access, and the L1
+// serves about one per cycle.  Here groups of G lanes load the P pieces of one record each (one
+// piece per lane and instruction), so every load instruction reads 64 / G whole records from
+// contiguous lines; the pieces then go through LDS to the lane that needs the record.  The LDS
+// slot of a record is S bytes (S / 16 odd multiples of four dwords keep the 16-lane phases of a
+// 16-byte LDS access on distinct banks).
+typedef unsigned piece16 __attribute__((ext_vector_type(4)));
+
+template <int P, int G, int S>
+struct CoopGather {
+  static_assert(P <= G && (G & (G - 1)) == 0 && G <= 64, "pieces per record");
+  static constexpr int R = 64 / G;  // records per load instruction
+  __device__ static void load(const char *base, size_t rec_bytes, int j, int lane,
+                              piece16 (&v)[G]) {
+    const int p = lane & (G - 1);
+    const int ps = p < P ? p : 0;  // lanes beyond the record's pieces reload piece 0
+#pragma unroll
+    for (int q = 0; q < G; q++) {
+      const int jm = __shfl(j, q * R + lane / G, 64);
+      v[q] = *(const piece16 *)(base + (size_t)jm * rec_bytes + 16 * ps);
+    }
+  }
+  __device__ static void store(char *lds, int lane, const piece16 (&v)[G]) {
+    const int p = lane & (G - 1);
+#pragma unroll
+    for (int q = 0; q < G; q++)
+      if (p < P) *(piece16 *)(lds + (q * R + lane / G) * S + 16 * p) = v[q];
+  }
+};
+
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// 1: the FP32 main pass gathers the neighbour records cooperatively through LDS
+#ifndef PAIR32_COOP
+#define PAIR32_COOP 1
+#endif
+
 // ---------------------------------------------------------------------------------------------
 // particle update after the neighbour sums (FP64 for both precisions)
 struct PairSums {
@@ -654,19 +696,22 @@ struct ToolVol {
   float tool, vol;
 };
 
-template <class Acc, int LPP>
+template <class Acc, bool COOP>
 __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
     k_pair32(unsigned n_init, State st, const Rec32 *__restrict__ rec, const int *__restrict__ nbr,
              int stride, const int *__restrict__ count, Consts c, const Counters *cnt) {
 #pragma clang fp contract(fast)
-  const unsigned gt = xcd_block() * BSP + threadIdx.x;
-  const unsigned i = gt / LPP;
-  const int sub = (int)(gt % LPP);
+  constexpr int LPP = 1;
+  const unsigned i = xcd_block() * BSP + threadIdx.x;
   const unsigned N = cnt->N;
-  if (i >= N || i >= n_init) return;
-  const uint32_t fl = st.flags[i];
-  if (fl & FL_BLANKED) return;
-  const Rec32 ri = rec[i];
+  const bool in_range = i < N && i < n_init;
+  const uint32_t fl = in_range ? st.flags[i] : FL_BLANKED;
+  const bool act = in_range && !(fl & FL_BLANKED);
+  // the cooperative gather needs every lane of the wavefront until the end of the pair loop;
+  // inactive lanes run it with an empty list on the record of particle 0
+  if (!COOP && !act) return;
+  const unsigned ic = act ? i : 0u;
+  const Rec32 ri = rec[ic];
   const bool tool_i = ri.tool == 1.f;
   const bool tool_nz = (fl & FL_TOOL_NZ) != 0;
   const double thermal_alpha = tool_nz ? c.alpha_tool : c.alpha_wp;
@@ -674,11 +719,9 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
   const float hi = ri.h;
   const float h1 = __builtin_amdgcn_rcpf(hi);
   const float fac = (float)INV_PI_D * h1 * h1 * h1;
-  const int n = count[i];
-  int k0, k1;
-  list_part<LPP>(n, sub, k0, k1);
-  const int *row = nbr + (size_t)i * stride + k0;
-  const int nl = k1 - k0;
+  const int n = act ? count[i] : 0;
+  const int *row = nbr + (size_t)ic * stride;
+  const int nl = n;
   const float mass = (float)c.i_mass;
 
   float iB[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};  // invB[col][row] flattened
@@ -797,7 +840,39 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
       Tl.add(2.0f * rj.vol * (Ti - rj.T) * rinv * e_w);
     }
   };
-  gather_loop<false, PAIR32_DEEP != 0>(row, nl, [&](int j) { return rec[j]; }, body);
+  if (COOP) {
+    typedef CoopGather<sizeof(Rec32) / 16, 8, sizeof(Rec32)> CG;
+    __shared__ __align__(16) char s_rec[2][BSP / 64][64 * sizeof(Rec32)];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int nmax = nl;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o, 64));
+    typedef int iv4 __attribute__((ext_vector_type(4)));
+    iv4 win = {0, 0, 0, 0};
+    // neighbour of this lane at step k (its own particle past the end of its list); k ascends
+    auto idx_at = [&](int k) -> int {
+      if ((k & 3) == 0 && k < nl) win = *(const iv4 *)(row + k);
+      if (k >= nl) return (int)ic;
+      const int q = k & 3;
+      return q == 0 ? win.x : q == 1 ? win.y : q == 2 ? win.z : win.w;
+    };
+    piece16 v[8];
+    if (nmax > 0) {
+      CG::load((const char *)rec, sizeof(Rec32), idx_at(0), lane, v);
+      CG::store(s_rec[0][w], lane, v);
+      wave_sync_lds();
+    }
+    for (int k = 0; k < nmax; k++) {
+      const bool more = k + 1 < nmax;
+      if (more) CG::load((const char *)rec, sizeof(Rec32), idx_at(k + 1), lane, v);
+      if (k < nl) body(*(const Rec32 *)(s_rec[k & 1][w] + lane * sizeof(Rec32)));
+      if (more) CG::store(s_rec[(k + 1) & 1][w], lane, v);
+      wave_sync_lds();
+    }
+  } else {
+    gather_loop<false, PAIR32_DEEP != 0>(row, nl, [&](int j) { return rec[j]; }, body);
+  }
+  if (!act) return;
   PairSums ps;
 #pragma unroll
   for (int a = 0; a < 3; a++)
@@ -812,7 +887,6 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
     ps.xt[a] = lanes_sum<LPP>(xt[a].val());
   }
   ps.Tl = lanes_sum<LPP>(Tl.val());
-  if (sub != 0) return;
   particle_update(st, i, fl, c, ps, brook, thermal_alpha);
 }
 
@@ -842,11 +916,11 @@ void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *re
     hipLaunchKernelGGL((k_pair64<PAIR_LPP>), grid, block, 0, s, n_init, st, (const Rec64 *)rec,
                        nbr, nl_stride, count, c, cnt);
   else if (c.precision == MFG_FP32_KAHAN)
-    hipLaunchKernelGGL((k_pair32<AccKahan, PAIR_LPP>), grid, block, 0, s, n_init, st,
-                       (const Rec32 *)rec, nbr, nl_stride, count, c, cnt);
+    hipLaunchKernelGGL((k_pair32<AccKahan, PAIR32_COOP != 0>), dim3(nb(n_init, BSP)), block, 0, s,
+                       n_init, st, (const Rec32 *)rec, nbr, nl_stride, count, c, cnt);
   else
-    hipLaunchKernelGGL((k_pair32<AccWide, PAIR_LPP>), grid, block, 0, s, n_init, st,
-                       (const Rec32 *)rec, nbr, nl_stride, count, c, cnt);
+    hipLaunchKernelGGL((k_pair32<AccWide, PAIR32_COOP != 0>), dim3(nb(n_init, BSP)), block, 0, s,
+                       n_init, st, (const Rec32 *)rec, nbr, nl_stride, count, c, cnt);
 }
 
 }  // namespace mfg
