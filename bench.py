@@ -32,8 +32,9 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3
 #   pair pass: 280 FP operations per interacting pair (CSPM + B 55, grad W 25, grad v 30,
 #              artificial stress 15, stress divergence 63, viscosity 40, XSPH 10, Brookshaw 25)
 #   neighbour pass: 11 per candidate (r^2 and support test) + 10 per candidate for the PSE sum
-#   algorithmic HBM bytes of the pair pass per particle: its own record and state read once,
-#   its neighbour list (4 B per pair), the updated state written once (FP64 field counts below)
+#   algorithmic HBM bytes of the pair pass per particle: its own record read once (192 B FP64,
+#   112 B FP32; every neighbour record is a record some particle reads anyway), flags and list
+#   length (8 B), its 25 FP64 neighbour sums written (200 B), and its list (4 B per pair)
 FLOPS_PER_PAIR = 280.0
 FLOPS_PER_CANDIDATE = 11.0
 FLOPS_PER_PSE_CANDIDATE = 10.0
@@ -86,14 +87,14 @@ def gpu_run(args):
 
 
 # algorithmic HBM bytes of one pair pass per particle, besides 4 B per neighbour-list entry:
-#   own pair record (192 B FP64 / 112 B FP32) + count 4 B + flags 4 B
-#   + start-of-step state read 264 B (x, v, S, S_t, v_t, rho, T, rho_t, T_t, fc, ft, eps_pl,
-#     eps_pl_dot) + new state and rates written 240 B
-PAIR_BYTES_PER_PARTICLE = {"fp64": 704.0, "fp32": 624.0}
+#   own pair record (192 B FP64 / 112 B FP32) + count 4 B + flags 4 B + 25 neighbour sums
+#   written (200 B); the particle update that reads the sums back is a kernel of its own
+PAIR_BYTES_PER_PARTICLE = {"fp64": 400.0, "fp32": 320.0}
 
 # committed rocprofv3 FETCH_SIZE / WRITE_SIZE summaries: (kernel, preset, scale) -> file
 TRAFFIC_PROFILES = {
-    ("pair_update_fp64", 2, 1.0): "pair_update_fp64_cut1M_single_grain_traffic.md",
+    ("pair_fp64", 2, 1.0): "pair_fp64_cut1M_single_grain_traffic.md",
+    ("pair_fp32_wide", 2, 1.0): "pair_fp32_wide_cut1M_single_grain_traffic.md",
 }
 
 
@@ -121,7 +122,7 @@ def roofline(res, args, pse: bool):
     fp64 = args.precision == "fp64"
     peak = FP64_VECTOR_PEAK_TFLOPS if fp64 else FP32_VECTOR_PEAK_TFLOPS
     n = res["n_active"]
-    if dom.startswith("pair_update"):
+    if dom.startswith("pair_"):
         flops = FLOPS_PER_PAIR * res["pairs"]
         basis = f"{FLOPS_PER_PAIR:.0f} flop x {res['pairs']} interacting pairs per launch"
         alg_bytes = PAIR_BYTES_PER_PARTICLE["fp64" if fp64 else "fp32"] * n + 4.0 * res["pairs"]

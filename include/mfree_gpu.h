@@ -98,7 +98,11 @@ typedef struct {
                          float (src/kernels.cuh:48, src/actions_gpu.cu:50,203,217,420-421,
                          src/interactions_gpu.cu:414-415); 0: full double everywhere */
   int max_neighbours; /* neighbour-list capacity per particle; 0 selects 256 */
+  int pair_pass;      /* MFG_PAIR_LIST (0): one thread per particle gathers its neighbour records;
+                         MFG_PAIR_TILED (1): one wavefront per grid column stages the records of
+                         the column's neighbour cells in LDS.  Same sums, same order. */
 } mfg_options;
+enum { MFG_PAIR_LIST = 0, MFG_PAIR_TILED = 1 };
 
 /* opaque handles */
 typedef struct mfg_particles mfg_particles;

@@ -41,7 +41,7 @@ struct Lib {
   bool ready = false;
   int device = -1;
   hipStream_t stream = nullptr;
-  mfg_options opt{0, MFG_HEAT_BROOKSHAW, 1, 256};
+  mfg_options opt{0, MFG_HEAT_BROOKSHAW, 1, 256, MFG_PAIR_LIST};
   // exhaustive contact: closest triangle found by one wavefront per (particle, tool) pair;
   // MFG_LANE_CONTACT=0 in the environment selects the one-thread-per-particle search instead
   bool opt_contact_search = true;
@@ -175,6 +175,8 @@ struct mfg_particles {
   void *rec = nullptr;
   double4 *nl4 = nullptr;
   double2 *pse2 = nullptr;
+  double *sums = nullptr;  // neighbour sums of the pair pass, PAIR_SUM_COUNT x sum_stride (SoA)
+  size_t sum_stride = 0;
   int *nbr = nullptr;
   int *count = nullptr;
   int nl_stride = 0;
@@ -277,7 +279,7 @@ size_t padded(size_t n) { return (n + 63) / 64 * 64; }
 
 void free_particles(mfg_particles *p) {
   void *ptrs[] = {p->arena[0], p->arena[1], p->uarena[0], p->uarena[1], p->rec, p->nl4, p->pse2,
-                  p->nbr, p->count, p->key, p->key_sorted, p->order, p->blk, p->cnt, p->partials,
+                  p->sums, p->nbr, p->count, p->key, p->key_sorted, p->order, p->blk, p->cnt, p->partials,
                   p->p_dbg, p->R_dbg, p->flag_d, p->stat_part, p->cs_list, p->cs_n,
                   p->cs_slot, p->cs_res};
   for (void *q : ptrs)
@@ -444,6 +446,8 @@ int check_counters(mfg_particles *p) {
     return fail(MFG_ERR_CAPACITY, "a particle has %d neighbours, list capacity is %d",
                 p->cnt_h->max_count, p->cap);
   if (p->cnt_h->err_grid) return fail(MFG_ERR_GRID, "unblanked particle outside the grid");
+  if (p->cnt_h->err_list)
+    return fail(MFG_ERR_STATE, "a neighbour list entry lies outside the particle's cell stencil");
   return MFG_OK;
 }
 
@@ -612,6 +616,8 @@ int mfg_synchronize(void) {
 int mfg_set_options(const mfg_options *o) {
   if (!o) return fail(MFG_ERR_ARG, "null options");
   if (o->heat < MFG_HEAT_NONE || o->heat > MFG_HEAT_PSE) return fail(MFG_ERR_ARG, "heat model %d", o->heat);
+  if (o->pair_pass != MFG_PAIR_LIST && o->pair_pass != MFG_PAIR_TILED)
+    return fail(MFG_ERR_ARG, "pair pass %d", o->pair_pass);
   g_lib.opt = *o;
   if (g_lib.opt.max_neighbours <= 0) g_lib.opt.max_neighbours = 256;
   // list rows are read as 16-byte vectors: a multiple of 4 entries, at least 8
@@ -726,10 +732,12 @@ int mfg_particle_gpu_create(unsigned N, const double *pos4, const double *vel4, 
   size_t rec_bytes = precision == MFG_FP64 ? sizeof(Rec64) : sizeof(Rec32);
   p->cap = g_lib.opt.max_neighbours > 0 ? g_lib.opt.max_neighbours : 256;
   p->nl_stride = p->cap;  // row-major lists: nbr[i * cap + k]
+  p->sum_stride = n;
   p->nblk = (int)((N + 255) / 256);
   p->contact_blocks = p->nblk;
   if (hipMalloc(&p->rec, rec_bytes * n) != hipSuccess || hipMalloc(&p->nl4, sizeof(double4) * n) != hipSuccess ||
       hipMalloc(&p->pse2, sizeof(double2) * n) != hipSuccess ||
+      hipMalloc(&p->sums, sizeof(double) * PAIR_SUM_COUNT * n) != hipSuccess ||
       hipMalloc(&p->nbr, sizeof(int) * (size_t)p->cap * n) != hipSuccess ||
       hipMalloc(&p->count, sizeof(int) * n) != hipSuccess || hipMalloc(&p->key, sizeof(int) * n) != hipSuccess ||
       hipMalloc(&p->key_sorted, sizeof(int) * n) != hipSuccess ||
@@ -1519,10 +1527,16 @@ int mfg_leap_frog_step(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int rec
     if (rc != MFG_OK) return rc;
   }
   {
-    ProfScope ps(p->precision == MFG_FP64          ? "pair_update_fp64"
-                 : p->precision == MFG_FP32_KAHAN ? "pair_update_fp32_kahan"
-                                                  : "pair_update_fp32_wide");
-    launch_pair(s, p->N_init, p->st(), p->rec, p->nbr, p->nl_stride, p->count, c, p->cnt);
+    ProfScope ps(p->precision == MFG_FP64          ? "pair_fp64"
+                 : p->precision == MFG_FP32_KAHAN ? "pair_fp32_kahan"
+                                                  : "pair_fp32_wide");
+    launch_pair(s, p->N_init, p->st(), p->rec, p->nbr, p->nl_stride, p->count, c, p->cnt,
+                g->info(), g->count, p->key_sorted, g_lib.opt.pair_pass == MFG_PAIR_TILED ? 1 : 0, p->sums,
+                p->sum_stride);
+  }
+  {
+    ProfScope ps("particle_update");
+    launch_particle_update(s, p->N_init, p->st(), p->sums, p->sum_stride, c, p->cnt);
   }
   // tools
   ToolSet ts;

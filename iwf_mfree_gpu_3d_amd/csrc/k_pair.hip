@@ -312,30 +312,6 @@ __device__ __forceinline__ void gather_loop(const int *row, int n, Load &&load, 
   }
 }
 
-// Lanes per particle of the pair pass.  Each of the LPP lanes of a particle takes one
-// contiguous, 4-aligned part of the neighbour list; the partial sums are then added across the
-// lanes (lane 0 + 1, then + the other pair).  Fewer particles in flight per wavefront keep the
-// neighbour records of the particles in flight within the L2 of the XCD.
-#ifndef PAIR_LPP
-#define PAIR_LPP 1
-#endif
-
-// part [k0, k1) of a list of n entries for lane `sub` of LPP (every part starts at a multiple of 4)
-template <int LPP>
-__device__ __forceinline__ void list_part(int n, int sub, int &k0, int &k1) {
-  const int q = ((n + 4 * LPP - 1) / (4 * LPP)) * 4;
-  k0 = min(sub * q, n);
-  k1 = min(k0 + q, n);
-}
-
-// sum of v over the LPP lanes of a particle (lanes sub = 0 .. LPP-1 are adjacent)
-template <int LPP>
-__device__ __forceinline__ double lanes_sum(double v) {
-  if (LPP >= 2) v += __shfl_xor(v, 1, 64);
-  if (LPP >= 4) v += __shfl_xor(v, 2, 64);
-  return v;
-}
-
 // Cooperative gather.  The 64 lanes of a wavefront each need one neighbour record per step.
 // Loaded lane by lane, every 16-byte piece of every record is a separate L1 access, and the L1
 // serves about one per cycle.  Here groups of G lanes load the P pieces of one record each (one
@@ -372,11 +348,6 @@ __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
-
-// 1: the FP32 main pass gathers the neighbour records cooperatively through LDS
-#ifndef PAIR32_COOP
-#define PAIR32_COOP 1
-#endif
 
 // ---------------------------------------------------------------------------------------------
 // particle update after the neighbour sums (FP64 for both precisions)
@@ -505,89 +476,121 @@ __device__ __forceinline__ void particle_update(const State &st, unsigned i, uin
   st.T[i] = T;
 }
 
-// ---------------------------------------------------------------------------------------------
-// FP64 pair pass
-template <int LPP>
-__global__ void __launch_bounds__(BSP, PAIR64_WAVES)
-    k_pair64(unsigned n_init, State st, const Rec64 *__restrict__ rec, const int *__restrict__ nbr,
-             int stride, const int *__restrict__ count, Consts c, const Counters *cnt) {
-  const unsigned gt = xcd_block() * BSP + threadIdx.x;
-  const unsigned i = gt / LPP;
-  const int sub = (int)(gt % LPP);
-  const unsigned N = cnt->N;
-  if (i >= N || i >= n_init) return;
-  const uint32_t fl = st.flags[i];
-  if (fl & FL_BLANKED) return;
-  const Rec64 ri = rec[i];
-  const bool tool_i = ri.tool == 1.;
-  const bool tool_nz = (fl & FL_TOOL_NZ) != 0;
-  const double thermal_alpha = tool_nz ? c.alpha_tool : c.alpha_wp;
-  const bool brook = c.heat == MFG_HEAT_BROOKSHAW && thermal_alpha != 0.;
-  const double hi = ri.h;
-  const double h1 = 1 / hi;
-  const double fac = (INV_PI_D)*h1 * h1 * h1;
-  const int n = count[i];
-  int k0, k1;
-  list_part<LPP>(n, sub, k0, k1);
-  const int *row = nbr + (size_t)i * stride + k0;
-  const int nl = k1 - k0;
-  const int single = c.single_sqrt;
-  const double mass = c.i_mass;
+// The pair kernels end by storing a particle's neighbour sums (SoA, PAIR_SUMS doubles per
+// particle: vd[9], sd[9], vt[3], xt[3], Tl); the per-particle kernel below reads them back and
+// runs the update.  Split this way, neither kernel carries the other's registers.
+constexpr int PAIR_SUMS = 25;
 
-  M3 invB;
+__device__ __forceinline__ void store_sums(double *__restrict__ sums, size_t stride, unsigned i,
+                                           const PairSums &ps) {
 #pragma unroll
   for (int a = 0; a < 3; a++)
 #pragma unroll
-    for (int b = 0; b < 3; b++) invB.m[a][b] = (a == b) ? 1. : 0.;
-  if (c.cspm) {
-    M3 B;
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-      for (int b = 0; b < 3; b++) B.m[a][b] = 0.;
-    if (!tool_nz) {
-      struct G {
-        double4 p, q;  // (x, y, z, h), (T, vol, c, tool)
-      };
-      auto load = [&](int j) { return G{*(const double4 *)&rec[j].x, *(const double4 *)&rec[j].T}; };
-      gather_loop(row, nl, load, [&](const G &g) {
-        if (g.q.w == 1.) return;
-        const double volj = g.q.y;
-        double xij = ri.x - g.p.x, yij = ri.y - g.p.y, zij = ri.z - g.p.z;
-        double rr2 = xij * xij + yij * yij + zij * zij;
-        double w, wx, wy, wz;
-        cubic_spline64(xij, yij, zij, rr2, h1, fac, single, w, wx, wy, wz);
-        B.m[0][0] -= volj * xij * wx; B.m[1][0] -= volj * xij * wy; B.m[2][0] -= volj * xij * wz;
-        B.m[0][1] -= volj * yij * wx; B.m[1][1] -= volj * yij * wy; B.m[2][1] -= volj * yij * wz;
-        B.m[0][2] -= volj * zij * wx; B.m[1][2] -= volj * zij * wy; B.m[2][2] -= volj * zij * wz;
-      });
+    for (int b = 0; b < 3; b++) {
+      sums[(3 * a + b) * stride + i] = ps.vd.m[a][b];
+      sums[(9 + 3 * a + b) * stride + i] = ps.sd.m[a][b];
     }
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-      for (int b = 0; b < 3; b++) B.m[a][b] = lanes_sum<LPP>(B.m[a][b]);
-    if (det3(B) > 1e-8) inv3(B, invB);
-  }
-
-  PairSums ps;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
+    sums[(18 + a) * stride + i] = ps.vt[a];
+    sums[(21 + a) * stride + i] = ps.xt[a];
+  }
+  sums[24 * stride + i] = ps.Tl;
+}
+
+// Jaumann rate, continuity, momentum, advection, invalidation, leap-frog correction and the
+// Johnson-Cook radial return of every particle the pair pass summed (not blanked, in a cell)
+__global__ void __launch_bounds__(BSP)
+    k_particle_update(unsigned n_init, State st, const double *__restrict__ sums, size_t stride,
+                      Consts c, const Counters *cnt) {
+  const unsigned i = blockIdx.x * BSP + threadIdx.x;
+  if (i >= cnt->N || i >= n_init) return;
+  const uint32_t fl = st.flags[i];
+  if (fl & FL_BLANKED) return;
+  PairSums ps;
+#pragma unroll
+  for (int a = 0; a < 3; a++)
 #pragma unroll
     for (int b = 0; b < 3; b++) {
-      ps.vd.m[a][b] = 0.;
-      ps.sd.m[a][b] = 0.;
+      ps.vd.m[a][b] = sums[(3 * a + b) * stride + i];
+      ps.sd.m[a][b] = sums[(9 + 3 * a + b) * stride + i];
     }
-    ps.vt[a] = 0.;
-    ps.xt[a] = 0.;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    ps.vt[a] = sums[(18 + a) * stride + i];
+    ps.xt[a] = sums[(21 + a) * stride + i];
   }
-  ps.Tl = 0.;
-  const double Ti = brook ? ri.T : 0.;
-  const double wdeltap = c.wdeltap, eps = c.xspheps;
-  auto body = [&](const Rec64 &rj) {
+  ps.Tl = sums[24 * stride + i];
+  const bool tool_nz = (fl & FL_TOOL_NZ) != 0;
+  const double thermal_alpha = tool_nz ? c.alpha_tool : c.alpha_wp;
+  const bool brook = c.heat == MFG_HEAT_BROOKSHAW && thermal_alpha != 0.;
+  particle_update(st, i, fl, c, ps, brook, thermal_alpha);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Neighbour sums of one particle, FP64: the reference's expressions in its order
+// (src/interactions_gpu.cu:152-471).  The list kernels and the tiled kernel below share it.
+struct Pair64 {
+  typedef Rec64 Rec;
+  // what the CSPM sum reads of a neighbour: (x, y, z, h) and (T, vol, c, tool), the 16-byte
+  // pieces 0, 1, 10 and 11 of a record
+  struct CRec {
+    double4 p, q;
+  };
+  static constexpr int CPIECES = 4;
+  __device__ static __forceinline__ int cpiece(int pp) { return pp < 2 ? pp : pp + 8; }
+
+  Rec64 ri;
+  bool tool_i, tool_nz, brook;
+  double thermal_alpha, hi, h1, fac, Ti;
+  M3 B, invB;
+  PairSums ps;
+
+  __device__ __forceinline__ Pair64(const Consts &c, const Rec64 &r, uint32_t fl) : ri(r) {
+    tool_i = ri.tool == 1.;
+    tool_nz = (fl & FL_TOOL_NZ) != 0;
+    thermal_alpha = tool_nz ? c.alpha_tool : c.alpha_wp;
+    brook = c.heat == MFG_HEAT_BROOKSHAW && thermal_alpha != 0.;
+    hi = ri.h;
+    h1 = 1 / hi;
+    fac = (INV_PI_D)*h1 * h1 * h1;
+    Ti = brook ? ri.T : 0.;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+#pragma unroll
+      for (int b = 0; b < 3; b++) {
+        invB.m[a][b] = (a == b) ? 1. : 0.;
+        B.m[a][b] = 0.;
+        ps.vd.m[a][b] = 0.;
+        ps.sd.m[a][b] = 0.;
+      }
+      ps.vt[a] = 0.;
+      ps.xt[a] = 0.;
+    }
+    ps.Tl = 0.;
+  }
+  // tool particles (tool != 0) keep the identity correction
+  __device__ __forceinline__ bool cspm_sum() const { return !tool_nz; }
+  __device__ __forceinline__ void cspm_term(const Consts &c, const CRec &g) {
+    if (g.q.w == 1.) return;
+    const double volj = g.q.y;
+    double xij = ri.x - g.p.x, yij = ri.y - g.p.y, zij = ri.z - g.p.z;
+    double rr2 = xij * xij + yij * yij + zij * zij;
+    double w, wx, wy, wz;
+    cubic_spline64(xij, yij, zij, rr2, h1, fac, c.single_sqrt, w, wx, wy, wz);
+    B.m[0][0] -= volj * xij * wx; B.m[1][0] -= volj * xij * wy; B.m[2][0] -= volj * xij * wz;
+    B.m[0][1] -= volj * yij * wx; B.m[1][1] -= volj * yij * wy; B.m[2][1] -= volj * yij * wz;
+    B.m[0][2] -= volj * zij * wx; B.m[1][2] -= volj * zij * wy; B.m[2][2] -= volj * zij * wz;
+  }
+  __device__ __forceinline__ void cspm_end() {
+    if (det3(B) > 1e-8) inv3(B, invB);
+  }
+  __device__ __forceinline__ void term(const Consts &c, const Rec64 &rj) {
+    const double mass = c.i_mass, wdeltap = c.wdeltap, eps = c.xspheps;
     double xij = ri.x - rj.x, yij = ri.y - rj.y, zij = ri.z - rj.z;
     double rr2 = xij * xij + yij * yij + zij * zij;
     double w, wx0, wy0, wz0;
-    cubic_spline64(xij, yij, zij, rr2, h1, fac, single, w, wx0, wy0, wz0);
+    cubic_spline64(xij, yij, zij, rr2, h1, fac, c.single_sqrt, w, wx0, wy0, wz0);
     double w_x, w_y, w_z;
     if (!c.cspm) {
       w_x = wx0; w_y = wy0; w_z = wz0;
@@ -648,28 +651,16 @@ __global__ void __launch_bounds__(BSP, PAIR64_WAVES)
         ps.Tl += 2.0 * (rj.vol) * (Ti - rj.T) * rij1 * (ex * w_x + ey * w_y + ez * w_z);
       }
     }
-  };
-  // two record buffers alternate: the gather of neighbour k + 1 is in flight during the
-  // arithmetic of neighbour k (the sums still run in neighbour-list order)
-  gather_loop<PAIR64_SINGLE != 0, PAIR64_DEEP != 0>(row, nl, [&](int j) { return rec[j]; }, body);
-#pragma unroll
-  for (int a = 0; a < 3; a++) {
-#pragma unroll
-    for (int b = 0; b < 3; b++) {
-      ps.vd.m[a][b] = lanes_sum<LPP>(ps.vd.m[a][b]);
-      ps.sd.m[a][b] = lanes_sum<LPP>(ps.sd.m[a][b]);
-    }
-    ps.vt[a] = lanes_sum<LPP>(ps.vt[a]);
-    ps.xt[a] = lanes_sum<LPP>(ps.xt[a]);
   }
-  ps.Tl = lanes_sum<LPP>(ps.Tl);
-  if (sub != 0) return;
-  particle_update(st, i, fl, c, ps, brook, thermal_alpha);
-}
+  __device__ __forceinline__ void store(double *sums, size_t stride, unsigned i) const {
+    store_sums(sums, stride, i, ps);
+  }
+};
 
 // ---------------------------------------------------------------------------------------------
-// FP32 pair pass.  Positions stay FP64 until the relative position is formed; the pair
-// arithmetic is FP32 (contracted to FMA); every neighbour sum is compensated, either
+// Neighbour sums of one particle, FP32 pair arithmetic.  Positions stay FP64 until the relative
+// position is formed; the pair arithmetic is FP32 (contracted to FMA); every neighbour sum is
+// compensated, either
 //   AccKahan  FP32 running sum with Kahan's correction term (4 operations per term), or
 //   AccWide   the FP32 term widened and added in FP64 (2 operations per term: on CDNA4 an FP64
 //             add issues at the FP32 rate; the summation error, n 2^-53 sum|t|, is below
@@ -696,63 +687,73 @@ struct ToolVol {
   float tool, vol;
 };
 
-template <class Acc, bool COOP>
-__global__ void __launch_bounds__(BSP, PAIR32_WAVES)
-    k_pair32(unsigned n_init, State st, const Rec32 *__restrict__ rec, const int *__restrict__ nbr,
-             int stride, const int *__restrict__ count, Consts c, const Counters *cnt) {
-#pragma clang fp contract(fast)
-  constexpr int LPP = 1;
-  const unsigned i = xcd_block() * BSP + threadIdx.x;
-  const unsigned N = cnt->N;
-  const bool in_range = i < N && i < n_init;
-  const uint32_t fl = in_range ? st.flags[i] : FL_BLANKED;
-  const bool act = in_range && !(fl & FL_BLANKED);
-  // the cooperative gather needs every lane of the wavefront until the end of the pair loop;
-  // inactive lanes run it with an empty list on the record of particle 0
-  if (!COOP && !act) return;
-  const unsigned ic = act ? i : 0u;
-  const Rec32 ri = rec[ic];
-  const bool tool_i = ri.tool == 1.f;
-  const bool tool_nz = (fl & FL_TOOL_NZ) != 0;
-  const double thermal_alpha = tool_nz ? c.alpha_tool : c.alpha_wp;
-  const bool brook = c.heat == MFG_HEAT_BROOKSHAW && thermal_alpha != 0.;
-  const float hi = ri.h;
-  const float h1 = __builtin_amdgcn_rcpf(hi);
-  const float fac = (float)INV_PI_D * h1 * h1 * h1;
-  const int n = act ? count[i] : 0;
-  const int *row = nbr + (size_t)ic * stride;
-  const int nl = n;
-  const float mass = (float)c.i_mass;
+template <class Acc>
+struct Pair32 {
+  typedef Rec32 Rec;
+  typedef double4 CRec;  // x, y, z, (tool, vol): the first 32 bytes of a record
+  static constexpr int CPIECES = 2;
+  __device__ static __forceinline__ int cpiece(int pp) { return pp; }
 
-  float iB[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};  // invB[col][row] flattened
-  if (c.cspm) {
-    Acc B[9];
+  Rec32 ri;
+  bool tool_i, tool_nz, brook;
+  double thermal_alpha;
+  float hi, h1, fac, Ti, mass, inv_wdp, eps, va, vb, ve2;
+  float iB[9];  // inverse CSPM matrix, [col][row] flattened
+  Acc B[9], vd[9], sd[9], vt[3], xt[3], Tl;
+
+  __device__ __forceinline__ Pair32(const Consts &c, const Rec32 &r, uint32_t fl) : ri(r) {
+    tool_i = ri.tool == 1.f;
+    tool_nz = (fl & FL_TOOL_NZ) != 0;
+    thermal_alpha = tool_nz ? c.alpha_tool : c.alpha_wp;
+    brook = c.heat == MFG_HEAT_BROOKSHAW && thermal_alpha != 0.;
+    hi = ri.h;
+    h1 = __builtin_amdgcn_rcpf(hi);
+    fac = (float)INV_PI_D * h1 * h1 * h1;
+    Ti = brook ? ri.T : 0.f;
+    mass = (float)c.i_mass;
+    inv_wdp = c.wdeltap > 0 ? (float)(1.0 / c.wdeltap) : 0.f;
+    eps = (float)c.xspheps;
+    va = (float)c.v_alpha;
+    vb = (float)c.v_beta;
+    ve2 = (float)(c.v_eta * c.v_eta);
 #pragma unroll
-    for (int q = 0; q < 9; q++) B[q].zero();
-    if (!tool_nz) {
-      // x, y, z, (tool, vol) are the first 32 bytes of a record
-      gather_loop(row, nl, [&](int j) { return *(const double4 *)&rec[j].x; }, [&](const double4 &g) {
-        const ToolVol tv = __builtin_bit_cast(ToolVol, g.w);
-        if (tv.tool == 1.f) return;
-        float xij = (float)(ri.x - g.x), yij = (float)(ri.y - g.y), zij = (float)(ri.z - g.z);
-        float rr2 = xij * xij + yij * yij + zij * zij;
-        float rad = __builtin_amdgcn_sqrtf(rr2);
-        float rinv = __builtin_amdgcn_rcpf(rad);
-        float q = rad * h1;
-        float der = (q > 1.f) ? -0.75f * (2.f - q) * (2.f - q) * h1 * rinv
-                              : -3.0f * q * (1.f - 0.75f * q) * h1 * rinv;
-        float dw = der * fac * tv.vol;
-        float gx = dw * xij, gy = dw * yij, gz = dw * zij;
-        B[0].add(-xij * gx); B[3].add(-xij * gy); B[6].add(-xij * gz);
-        B[1].add(-yij * gx); B[4].add(-yij * gy); B[7].add(-yij * gz);
-        B[2].add(-zij * gx); B[5].add(-zij * gy); B[8].add(-zij * gz);
-      });
+    for (int q = 0; q < 9; q++) {
+      iB[q] = (q % 4 == 0) ? 1.f : 0.f;
+      B[q].zero();
+      vd[q].zero();
+      sd[q].zero();
     }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      vt[a].zero();
+      xt[a].zero();
+    }
+    Tl.zero();
+  }
+  __device__ __forceinline__ bool cspm_sum() const { return !tool_nz; }
+  __device__ __forceinline__ void cspm_term(const Consts &, const double4 &g) {
+#pragma clang fp contract(fast)
+    const ToolVol tv = __builtin_bit_cast(ToolVol, g.w);
+    if (tv.tool == 1.f) return;
+    float xij = (float)(ri.x - g.x), yij = (float)(ri.y - g.y), zij = (float)(ri.z - g.z);
+    float rr2 = xij * xij + yij * yij + zij * zij;
+    float rad = __builtin_amdgcn_sqrtf(rr2);
+    float rinv = __builtin_amdgcn_rcpf(rad);
+    float q = rad * h1;
+    float der = (q > 1.f) ? -0.75f * (2.f - q) * (2.f - q) * h1 * rinv
+                          : -3.0f * q * (1.f - 0.75f * q) * h1 * rinv;
+    float dw = der * fac * tv.vol;
+    float gx = dw * xij, gy = dw * yij, gz = dw * zij;
+    B[0].add(-xij * gx); B[3].add(-xij * gy); B[6].add(-xij * gz);
+    B[1].add(-yij * gx); B[4].add(-yij * gy); B[7].add(-yij * gz);
+    B[2].add(-zij * gx); B[5].add(-zij * gy); B[8].add(-zij * gz);
+  }
+  __device__ __forceinline__ void cspm_end() {
     M3 Bd, inv;
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
-      for (int b = 0; b < 3; b++) Bd.m[a][b] = lanes_sum<LPP>(B[3 * a + b].val());
+      for (int b = 0; b < 3; b++) Bd.m[a][b] = B[3 * a + b].val();
     if (det3(Bd) > 1e-8) {
       inv3(Bd, inv);
 #pragma unroll
@@ -761,24 +762,8 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
         for (int b = 0; b < 3; b++) iB[3 * a + b] = (float)inv.m[a][b];
     }
   }
-
-  Acc vd[9], sd[9], vt[3], xt[3], Tl;
-#pragma unroll
-  for (int q = 0; q < 9; q++) {
-    vd[q].zero();
-    sd[q].zero();
-  }
-#pragma unroll
-  for (int a = 0; a < 3; a++) {
-    vt[a].zero();
-    xt[a].zero();
-  }
-  Tl.zero();
-  const float Ti = brook ? ri.T : 0.f;
-  const float inv_wdp = c.wdeltap > 0 ? (float)(1.0 / c.wdeltap) : 0.f;
-  const float eps = (float)c.xspheps, va = (float)c.v_alpha, vb = (float)c.v_beta;
-  const float ve2 = (float)(c.v_eta * c.v_eta);
-  auto body = [&](const Rec32 &rj) {
+  __device__ __forceinline__ void term(const Consts &, const Rec32 &rj) {
+#pragma clang fp contract(fast)
     float xij = (float)(ri.x - rj.x), yij = (float)(ri.y - rj.y), zij = (float)(ri.z - rj.z);
     float rr2 = xij * xij + yij * yij + zij * zij;
     float rad = __builtin_amdgcn_sqrtf(rr2);
@@ -839,55 +824,342 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
       float e_w = (xij * w_x + yij * w_y + zij * w_z) * rinv;
       Tl.add(2.0f * rj.vol * (Ti - rj.T) * rinv * e_w);
     }
-  };
-  if (COOP) {
-    typedef CoopGather<sizeof(Rec32) / 16, 8, sizeof(Rec32)> CG;
-    __shared__ __align__(16) char s_rec[2][BSP / 64][64 * sizeof(Rec32)];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int nmax = nl;
+  }
+  __device__ __forceinline__ void store(double *sums, size_t stride, unsigned i) const {
+    PairSums ps;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o, 64));
-    typedef int iv4 __attribute__((ext_vector_type(4)));
-    iv4 win = {0, 0, 0, 0};
-    // neighbour of this lane at step k (its own particle past the end of its list); k ascends
-    auto idx_at = [&](int k) -> int {
-      if ((k & 3) == 0 && k < nl) win = *(const iv4 *)(row + k);
-      if (k >= nl) return (int)ic;
-      const int q = k & 3;
-      return q == 0 ? win.x : q == 1 ? win.y : q == 2 ? win.z : win.w;
-    };
-    piece16 v[8];
-    if (nmax > 0) {
-      CG::load((const char *)rec, sizeof(Rec32), idx_at(0), lane, v);
-      CG::store(s_rec[0][w], lane, v);
-      wave_sync_lds();
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) {
+        ps.vd.m[a][b] = vd[3 * a + b].val();
+        ps.sd.m[a][b] = sd[3 * a + b].val();
+      }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      ps.vt[a] = vt[a].val();
+      ps.xt[a] = xt[a].val();
     }
-    for (int k = 0; k < nmax; k++) {
-      const bool more = k + 1 < nmax;
-      if (more) CG::load((const char *)rec, sizeof(Rec32), idx_at(k + 1), lane, v);
-      if (k < nl) body(*(const Rec32 *)(s_rec[k & 1][w] + lane * sizeof(Rec32)));
-      if (more) CG::store(s_rec[(k + 1) & 1][w], lane, v);
-      wave_sync_lds();
+    ps.Tl = Tl.val();
+    store_sums(sums, stride, i, ps);
+  }
+};
+
+// ---------------------------------------------------------------------------------------------
+// List-gather pair pass: one thread per particle, each neighbour record gathered from memory
+// where it is used (two record buffers in flight, gather_loop).  Kept beside the tiled pass as
+// the reference point of its measurements and for grids the tiled pass does not take.
+template <class Body>
+__global__ void __launch_bounds__(BSP, PAIR64_WAVES)
+    k_pair_list(unsigned n_init, State st, const typename Body::Rec *__restrict__ rec,
+                const int *__restrict__ nbr, int stride, const int *__restrict__ count, Consts c,
+                const Counters *cnt, double *__restrict__ sums, size_t sstride) {
+  typedef typename Body::Rec Rec;
+  typedef typename Body::CRec CRec;
+  const unsigned i = xcd_block() * BSP + threadIdx.x;
+  const unsigned N = cnt->N;
+  if (i >= N || i >= n_init) return;
+  const uint32_t fl = st.flags[i];
+  if (fl & FL_BLANKED) return;
+  Body P(c, rec[i], fl);
+  const int n = count[i];
+  const int *row = nbr + (size_t)i * stride;
+  if (c.cspm) {
+    if (P.cspm_sum()) {
+      auto load = [&](int j) {
+        const char *b = (const char *)(rec + j);
+        CRec g;
+        piece16 *d = (piece16 *)&g;
+#pragma unroll
+        for (int pp = 0; pp < Body::CPIECES; pp++) d[pp] = *(const piece16 *)(b + 16 * Body::cpiece(pp));
+        return g;
+      };
+      gather_loop(row, n, load, [&](const CRec &g) { P.cspm_term(c, g); });
     }
-  } else {
-    gather_loop<false, PAIR32_DEEP != 0>(row, nl, [&](int j) { return rec[j]; }, body);
+    P.cspm_end();
+  }
+  gather_loop<PAIR64_SINGLE != 0, PAIR64_DEEP != 0>(row, n, [&](int j) { return rec[j]; },
+                                                    [&](const Rec &rj) { P.term(c, rj); });
+  P.store(sums, sstride, i);
+}
+
+// FP32 list pass with the cooperative gather for the main sum: the 64 lanes of a wavefront run
+// the pair loop together (inactive lanes with an empty list) because every lane helps load.
+template <class Acc>
+__global__ void __launch_bounds__(BSP, PAIR32_WAVES)
+    k_pair32_coop(unsigned n_init, State st, const Rec32 *__restrict__ rec,
+                  const int *__restrict__ nbr, int stride, const int *__restrict__ count,
+                  Consts c, const Counters *cnt, double *__restrict__ sums, size_t sstride) {
+  const unsigned i = xcd_block() * BSP + threadIdx.x;
+  const unsigned N = cnt->N;
+  const bool in_range = i < N && i < n_init;
+  const uint32_t fl = in_range ? st.flags[i] : FL_BLANKED;
+  const bool act = in_range && !(fl & FL_BLANKED);
+  const unsigned ic = act ? i : 0u;
+  Pair32<Acc> P(c, rec[ic], fl);
+  const int nl = act ? count[i] : 0;
+  const int *row = nbr + (size_t)ic * stride;
+  if (c.cspm) {
+    if (P.cspm_sum())
+      gather_loop(row, nl, [&](int j) { return *(const double4 *)&rec[j].x; },
+                  [&](const double4 &g) { P.cspm_term(c, g); });
+    P.cspm_end();
+  }
+  typedef CoopGather<sizeof(Rec32) / 16, 8, sizeof(Rec32)> CG;
+  __shared__ __align__(16) char s_rec[2][BSP / 64][64 * sizeof(Rec32)];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int nmax = nl;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o, 64));
+  typedef int iv4 __attribute__((ext_vector_type(4)));
+  iv4 win = {0, 0, 0, 0};
+  // neighbour of this lane at step k (its own particle past the end of its list); k ascends
+  auto idx_at = [&](int k) -> int {
+    if ((k & 3) == 0 && k < nl) win = *(const iv4 *)(row + k);
+    if (k >= nl) return (int)ic;
+    const int q = k & 3;
+    return q == 0 ? win.x : q == 1 ? win.y : q == 2 ? win.z : win.w;
+  };
+  piece16 v[8];
+  if (nmax > 0) {
+    CG::load((const char *)rec, sizeof(Rec32), idx_at(0), lane, v);
+    CG::store(s_rec[0][w], lane, v);
+    wave_sync_lds();
+  }
+  for (int k = 0; k < nmax; k++) {
+    const bool more = k + 1 < nmax;
+    if (more) CG::load((const char *)rec, sizeof(Rec32), idx_at(k + 1), lane, v);
+    if (k < nl) P.term(c, *(const Rec32 *)(s_rec[k & 1][w] + lane * sizeof(Rec32)));
+    if (more) CG::store(s_rec[(k + 1) & 1][w], lane, v);
+    wave_sync_lds();
   }
   if (!act) return;
-  PairSums ps;
-#pragma unroll
-  for (int a = 0; a < 3; a++)
-#pragma unroll
-    for (int b = 0; b < 3; b++) {
-      ps.vd.m[a][b] = lanes_sum<LPP>(vd[3 * a + b].val());
-      ps.sd.m[a][b] = lanes_sum<LPP>(sd[3 * a + b].val());
+  P.store(sums, sstride, i);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Cell-tiled pair pass.  One wavefront per grid column (gi, gj); the column's particles are
+// taken in chunks of at most 64 consecutive particles, one per lane.  Every neighbour of a chunk
+// lies in the 25 stencil columns (gi-2..gi+2, gj-2..gj+2) between cells kz_lo-2 and kz_hi+2 of
+// the chunk's first and last particle; in each stencil column those cells hold ONE contiguous
+// range of particles (a run), and the runs in (ii, jj) order concatenate to the chunk's staging
+// sequence S, ascending in particle index.  A neighbour list is ascending in particle index as
+// well (it follows the same cell order), so each lane walks S forwards.
+//
+// S streams through an LDS ring of NB blocks of B records: the whole wavefront loads a block
+// with 16-byte pieces that are contiguous in memory within a run (each record is read once per
+// chunk instead of once per pair), a block is refilled once no lane needs it any more, and the
+// load of the next block is in flight while the lanes compute.  A lane whose next neighbour is
+// not staged yet sits out a step.  Lanes work through their lists in list order, so every sum
+// has the order of the list pass.
+//
+// Staged records are P pieces of 16 bytes: piece pp of a staged record is the 16-byte piece
+// pofs(pp) of the record in memory (all of a record for the main sum, the CSPM fields for the
+// CSPM sum).
+#ifndef TILE64_B
+#define TILE64_B 16
+#endif
+#ifndef TILE64_NB
+#define TILE64_NB 8
+#endif
+#ifndef TILE32_B
+#define TILE32_B 16
+#endif
+#ifndef TILE32_NB
+#define TILE32_NB 12
+#endif
+#ifndef TILE_WAVES
+#define TILE_WAVES 2
+#endif
+
+struct TileRuns {
+  int rs[32], re[32], rp[32];  // run start / end (particle index), position of the run in S
+};
+
+template <int P, int B, int NB, class PieceOf, class Term>
+__device__ __forceinline__ void tile_stream(const char *__restrict__ base, int rec_bytes,
+                                            char *ring, const TileRuns &tr, int nr, int T,
+                                            const int *row, int nl, int cap, int lane, int &err,
+                                            PieceOf pofs, Term &&term) {
+  typedef int iv4 __attribute__((ext_vector_type(4)));
+  constexpr int RS = 16 * P;              // staged record bytes
+  constexpr int PPL = (P * B + 63) / 64;  // pieces per lane per block
+  static_assert(B <= 64 && (B & (B - 1)) == 0, "block of at most 64 records");
+  const int nblk = (T + B - 1) / B;
+  // this lane's list cursor: entry k, its group of four in win, the next group in nwin, the run
+  // r holding entry k and its position pos in S (rows hold cap >= 8 entries, a multiple of 4)
+  int k = 0, r = 0, pos = INT_MAX;
+  iv4 win = *(const iv4 *)row;
+  iv4 nwin;
+  auto locate = [&]() {
+    pos = INT_MAX;
+    if (k >= nl) return;
+    const int q = k & 3;
+    const int j = q == 0 ? win.x : q == 1 ? win.y : q == 2 ? win.z : win.w;
+    while (r < nr - 1 && j >= tr.re[r]) r++;
+    if (j >= tr.rs[r] && j < tr.re[r]) {
+      pos = j - tr.rs[r] + tr.rp[r];
+    } else {  // not in the chunk's stencil: the lane stops and the error is reported
+      err = 1;
+      k = nl;
     }
+  };
+  locate();
+  int sum = nl, mp = pos;
 #pragma unroll
-  for (int a = 0; a < 3; a++) {
-    ps.vt[a] = lanes_sum<LPP>(vt[a].val());
-    ps.xt[a] = lanes_sum<LPP>(xt[a].val());
+  for (int o = 32; o > 0; o >>= 1) {
+    sum += __shfl_xor(sum, o, 64);
+    mp = min(mp, __shfl_xor(mp, o, 64));
   }
-  ps.Tl = lanes_sum<LPP>(Tl.val());
-  particle_update(st, i, fl, c, ps, brook, thermal_alpha);
+  if (mp == INT_MAX) return;  // uniform: no lane has a neighbour
+  // particle index of record `lane` of block b (lanes < B and inside S)
+  auto block_index = [&](int b) -> int {
+    const int s = b * B + (lane & (B - 1));
+    int lo = 0, hi = nr - 1;  // last run whose position is <= s (a non-empty one)
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tr.rp[mid] <= s) lo = mid;
+      else hi = mid - 1;
+    }
+    return tr.rs[lo] + (s - tr.rp[lo]);
+  };
+  // a block's pieces are loaded into registers (contiguous in memory within a run) and written
+  // to its ring slot one step later.  Every lane issues every load (lanes without a piece of the
+  // block reload the block's first piece), so the loads need no branch around them.
+  piece16 v[PPL];
+  auto issue = [&](int b) {
+    const int jl = block_index(b);
+    const int nrec = min(B, T - b * B);
+#pragma unroll
+    for (int m = 0; m < PPL; m++) {
+      const int q = lane + 64 * m;
+      const int rr = q / P, pp = q - rr * P;
+      const bool own = q < P * B && rr < nrec;
+      const int jr = __shfl(jl, own ? rr : 0, 64);
+      v[m] = *(const piece16 *)(base + (size_t)jr * rec_bytes + 16 * pofs(own ? pp : 0));
+    }
+  };
+  // slot NB of the ring takes a block that is not needed (the ring was full when it was loaded)
+  auto land = [&](int b, int slot_idx) {
+    char *slot = ring + slot_idx * (B * RS);
+    const int nrec = min(B, T - b * B);
+#pragma unroll
+    for (int m = 0; m < PPL; m++) {
+      const int q = lane + 64 * m;
+      if (q < P * B && q / P < nrec) *(piece16 *)(slot + 16 * q) = v[m];
+    }
+  };
+  int blo = mp / B, bhi = blo;  // blocks [blo, bhi) are staged
+  issue(bhi);
+  land(bhi, bhi % NB);
+  bhi++;
+  bool pending = false;
+  int loaded = bhi;  // block held in v
+  // Each step issues the same memory operations in the same order, all before its arithmetic:
+  // the lane's next group of neighbour indices, then one block (the next block, or a repeat of
+  // the last one when the ring is full).  The block is the youngest load when it lands at the
+  // start of the next step, and the arithmetic of a whole step covers its latency.
+  // Every step either moves a lane on or stages the block the lowest lane waits for (within two
+  // steps), so 3 steps per list entry and 2 per block bound the loop.
+  const int max_steps = 3 * sum + 2 * nblk + 8;
+  int step = 0;
+  for (; step < max_steps; step++) {
+    land(loaded, pending ? bhi % NB : NB);
+    if (pending) bhi++;
+    while (blo < bhi && __ballot(pos < (blo + 1) * B) == 0) blo++;
+    if (__ballot(pos != INT_MAX) == 0) break;
+    if (blo == bhi) {  // no staged block is needed: go to the first block a lane waits for
+      int m2 = pos;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) m2 = min(m2, __shfl_xor(m2, o, 64));
+      blo = bhi = m2 / B;
+    }
+    pending = bhi < nblk && bhi - blo < NB;
+    loaded = min(bhi, nblk - 1);
+    nwin = *(const iv4 *)(row + min((k & ~3) + 4, cap - 4));
+    issue(loaded);
+    wave_sync_lds();  // the landed block is visible to every lane
+    if (pos < bhi * B) {
+      term(ring + ((pos / B) % NB) * (B * RS) + (pos % B) * RS);
+      k++;
+      if ((k & 3) == 0) win = nwin;
+      locate();
+    }
+  }
+  if (step == max_steps) err = 1;
+  wave_sync_lds();  // every read of the ring has completed before it is restaged
+}
+
+template <class Body, int B, int NB>
+__global__ void __launch_bounds__(64, TILE_WAVES)
+    k_pair_tiled(GridInfo g, const int *__restrict__ off, const int *__restrict__ key_sorted,
+                 State st, const typename Body::Rec *__restrict__ rec,
+                 const int *__restrict__ nbr, int stride, const int *__restrict__ count,
+                 Consts c, Counters *cnt, double *__restrict__ sums, size_t sstride) {
+  typedef typename Body::Rec Rec;
+  typedef typename Body::CRec CRec;
+  constexpr int RS = sizeof(Rec);
+  __shared__ __align__(16) char ring[(NB + 1) * B * RS];
+  __shared__ TileRuns tr;
+  const int lane = threadIdx.x;
+  const long col = xcd_block();
+  if (col >= (long)g.nx * g.ny) return;
+  const int gi = (int)(col / g.ny), gj = (int)(col % g.ny);
+  const long cbase = col * g.nz;
+  const int c0 = off[cbase], c1 = off[cbase + g.nz];
+  const int len = c1 - c0;
+  if (len <= 0) return;
+  const int lo_i = max(gi - 2, 0), hi_i = min(gi + 3, g.nx);
+  const int lo_j = max(gj - 2, 0), hi_j = min(gj + 3, g.ny);
+  const int nj = hi_j - lo_j, nr = (hi_i - lo_i) * nj;
+  const int nch = (len + 63) / 64;
+  int err = 0;
+  for (int ch = 0; ch < nch; ch++) {
+    const int p0 = c0 + (int)((long)ch * len / nch);
+    const int p1 = c0 + (int)((long)(ch + 1) * len / nch);
+    const int kz_lo = max(key_sorted[p0] - (int)cbase - 2, 0);
+    const int kz_hi = min(key_sorted[p1 - 1] - (int)cbase + 2, g.nz - 1);
+    int rs = 0, rl = 0;
+    if (lane < nr) {
+      const int ii = lo_i + lane / nj, jj = lo_j + lane % nj;
+      const long cb = ((long)ii * g.ny + jj) * g.nz;
+      rs = off[cb + kz_lo];
+      rl = off[cb + kz_hi + 1] - rs;
+    }
+    int incl = rl;
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+      const int t = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += t;
+    }
+    const int T = __builtin_amdgcn_readfirstlane(__shfl(incl, 31, 64));
+    if (lane < 32) {
+      tr.rs[lane] = rs;
+      tr.re[lane] = rs + rl;
+      tr.rp[lane] = incl - rl;
+    }
+    wave_sync_lds();
+    const unsigned i = (unsigned)(p0 + lane);
+    const bool in = lane < p1 - p0;
+    const uint32_t fl = in ? st.flags[i] : FL_BLANKED;
+    const bool act = in && !(fl & FL_BLANKED);
+    const unsigned ic = act ? i : (unsigned)p0;
+    Body P(c, rec[ic], fl);
+    const int n = act ? count[i] : 0;
+    const int *row = nbr + (size_t)ic * stride;
+    if (c.cspm) {
+      tile_stream<Body::CPIECES, B, NB>(
+          (const char *)rec, RS, ring, tr, nr, T, row, P.cspm_sum() ? n : 0, stride, lane, err,
+          [](int pp) { return Body::cpiece(pp); },
+          [&](const char *s) { P.cspm_term(c, *(const CRec *)s); });
+      P.cspm_end();
+    }
+    tile_stream<RS / 16, B, NB>((const char *)rec, RS, ring, tr, nr, T, row, n, stride, lane, err,
+                                [](int pp) { return pp; },
+                                [&](const char *s) { P.term(c, *(const Rec *)s); });
+    if (act) P.store(sums, sstride, i);
+    wave_sync_lds();  // the next chunk rewrites the runs and the ring
+  }
+  if (__ballot(err) != 0 && lane == 0) atomicOr(&cnt->err_list, 1);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -910,17 +1182,41 @@ void launch_neighbours(hipStream_t s, const GridInfo &g, const int *offsets, con
 }
 
 void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *rec, const int *nbr,
-                 int nl_stride, const int *count, const Consts &c, const Counters *cnt) {
-  const dim3 grid(nb((long)n_init * PAIR_LPP, BSP)), block(BSP);
+                 int nl_stride, const int *count, const Consts &c, Counters *cnt,
+                 const GridInfo &g, const int *offsets, const int *key_sorted, int tiled,
+                 double *sums, size_t sstride) {
+  if (tiled) {
+    const dim3 grid((unsigned)((long)g.nx * g.ny)), block(64);
+    if (c.precision == MFG_FP64)
+      hipLaunchKernelGGL((k_pair_tiled<Pair64, TILE64_B, TILE64_NB>), grid, block, 0, s, g,
+                         offsets, key_sorted, st, (const Rec64 *)rec, nbr, nl_stride, count, c,
+                         cnt, sums, sstride);
+    else if (c.precision == MFG_FP32_KAHAN)
+      hipLaunchKernelGGL((k_pair_tiled<Pair32<AccKahan>, TILE32_B, TILE32_NB>), grid, block, 0, s,
+                         g, offsets, key_sorted, st, (const Rec32 *)rec, nbr, nl_stride, count, c,
+                         cnt, sums, sstride);
+    else
+      hipLaunchKernelGGL((k_pair_tiled<Pair32<AccWide>, TILE32_B, TILE32_NB>), grid, block, 0, s,
+                         g, offsets, key_sorted, st, (const Rec32 *)rec, nbr, nl_stride, count, c,
+                         cnt, sums, sstride);
+    return;
+  }
+  const dim3 grid(nb(n_init, BSP)), block(BSP);
   if (c.precision == MFG_FP64)
-    hipLaunchKernelGGL((k_pair64<PAIR_LPP>), grid, block, 0, s, n_init, st, (const Rec64 *)rec,
-                       nbr, nl_stride, count, c, cnt);
+    hipLaunchKernelGGL((k_pair_list<Pair64>), grid, block, 0, s, n_init, st, (const Rec64 *)rec,
+                       nbr, nl_stride, count, c, cnt, sums, sstride);
   else if (c.precision == MFG_FP32_KAHAN)
-    hipLaunchKernelGGL((k_pair32<AccKahan, PAIR32_COOP != 0>), dim3(nb(n_init, BSP)), block, 0, s,
-                       n_init, st, (const Rec32 *)rec, nbr, nl_stride, count, c, cnt);
+    hipLaunchKernelGGL((k_pair32_coop<AccKahan>), grid, block, 0, s, n_init, st,
+                       (const Rec32 *)rec, nbr, nl_stride, count, c, cnt, sums, sstride);
   else
-    hipLaunchKernelGGL((k_pair32<AccWide, PAIR32_COOP != 0>), dim3(nb(n_init, BSP)), block, 0, s,
-                       n_init, st, (const Rec32 *)rec, nbr, nl_stride, count, c, cnt);
+    hipLaunchKernelGGL((k_pair32_coop<AccWide>), grid, block, 0, s, n_init, st,
+                       (const Rec32 *)rec, nbr, nl_stride, count, c, cnt, sums, sstride);
+}
+
+void launch_particle_update(hipStream_t s, unsigned n_init, const State &st, const double *sums,
+                            size_t sstride, const Consts &c, const Counters *cnt) {
+  hipLaunchKernelGGL(k_particle_update, dim3(nb(n_init, BSP)), dim3(BSP), 0, s, n_init, st, sums,
+                     sstride, c, cnt);
 }
 
 }  // namespace mfg

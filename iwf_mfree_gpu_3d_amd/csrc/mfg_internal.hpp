@@ -10,6 +10,8 @@
 //     pse2  : double2 (T, rho)                    -- PSE heat pass (only with PSE)
 //     rec   : pair record, 192 B (FP64) / 112 B (FP32) -- everything a neighbour contributes
 //   neighbour list: int32 nbr[i * cap + k] (row-major), count[i]
+//   neighbour sums: 25 doubles per particle (SoA), written by the pair pass and read by the
+//     per-particle update
 //   grid: key[N_init] (unsorted), key_sorted[N_init], order[N_init], cell counts / offsets
 //         [num_cell + 1], fill counters [num_cell]
 #pragma once
@@ -104,7 +106,7 @@ struct Counters {
   int err_capacity;      // neighbour list overflow
   int err_grid;          // unblanked particle outside the grid
   int max_count;         // largest neighbour count of the last list build
-  int pad;
+  int err_list;          // tiled pair pass: a list entry outside the particle's stencil
   unsigned long long candidates;  // candidate pairs visited by the last neighbour pass
   unsigned long long pairs;       // interacting pairs of the last neighbour pass
 };
@@ -182,8 +184,19 @@ void launch_neighbours(hipStream_t s, const GridInfo &g, const int *offsets, con
                        const double2 *pse2, const double *h, const uint32_t *flags,
                        const Consts &c, int *nbr, int cap, int *count, double *T_t, Counters *cnt,
                        int want_pse, unsigned long long *stat_part);
+// pair pass (neighbour sums).  tiled: one wavefront per grid column stages the neighbour
+// records of the column's particles in LDS (offsets and sorted keys of the grid); otherwise one
+// thread per particle gathers them from memory
 void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *rec, const int *nbr,
-                 int nl_stride, const int *count, const Consts &c, const Counters *cnt);
+                 int nl_stride, const int *count, const Consts &c, Counters *cnt,
+                 const GridInfo &g, const int *offsets, const int *key_sorted, int tiled,
+                 double *sums, size_t sum_stride);
+// neighbour sums per particle written by the pair pass (SoA, sum_stride apart)
+constexpr int PAIR_SUM_COUNT = 25;
+// Jaumann, continuity, momentum, advection, invalidation, leap-frog correction and Johnson-Cook
+// radial return from the neighbour sums
+void launch_particle_update(hipStream_t s, unsigned n_init, const State &st, const double *sums,
+                            size_t sum_stride, const Consts &c, const Counters *cnt);
 
 void launch_tool_update(hipStream_t s, Tri *tris, const double *p_init, int n, double t,
                         const double vel[3]);
