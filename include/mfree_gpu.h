@@ -101,6 +101,11 @@ typedef struct {
   int pair_pass;      /* MFG_PAIR_LIST (0): one thread per particle gathers its neighbour records;
                          MFG_PAIR_TILED (1): one wavefront per grid column stages the records of
                          the column's neighbour cells in LDS.  Same sums, same order. */
+  int pair_strip;     /* list pair pass: particles are visited in strips of this many grid
+                         columns along y, x-row by x-row inside a strip, which keeps the
+                         neighbour records in flight within an XCD's L2; 0 selects 8 (FP64) or
+                         16 (FP32 modes), a negative value the sorted order.  The sums do not
+                         depend on it. */
 } mfg_options;
 enum { MFG_PAIR_LIST = 0, MFG_PAIR_TILED = 1 };
 

@@ -82,12 +82,13 @@ MFG_PAIR_LIST, MFG_PAIR_TILED = 0, 1
 
 
 def set_options(cspm: bool = False, heat: int = MFG_HEAT_BROOKSHAW, single_sqrt: bool = True,
-                max_neighbours: int = 0, pair_pass: int = MFG_PAIR_LIST) -> None:
-    """Switches that are compile-time macros in the reference (src/types.h:32-108), and the
-    pair-pass kernel (list gather or cell-tiled: the same sums in the same order)."""
+                max_neighbours: int = 0, pair_pass: int = MFG_PAIR_LIST, pair_strip: int = 0) -> None:
+    """Switches that are compile-time macros in the reference (src/types.h:32-108), the
+    pair-pass kernel (list gather or cell-tiled: the same sums in the same order) and the list
+    pass's visiting order (strip width in grid columns; 0 default, negative the sorted order)."""
     _lib.init()
     check(_L().mfg_set_options(C.byref(_lib.options(int(cspm), int(heat), int(single_sqrt), int(max_neighbours),
-                                                    int(pair_pass)))))
+                                                    int(pair_pass), int(pair_strip)))))
 
 
 def actions_setup_physical_constants(c: phys_constants) -> None:

@@ -41,7 +41,7 @@ struct Lib {
   bool ready = false;
   int device = -1;
   hipStream_t stream = nullptr;
-  mfg_options opt{0, MFG_HEAT_BROOKSHAW, 1, 256, MFG_PAIR_LIST};
+  mfg_options opt{0, MFG_HEAT_BROOKSHAW, 1, 256, MFG_PAIR_LIST, 0};
   // exhaustive contact: closest triangle found by one wavefront per (particle, tool) pair;
   // MFG_LANE_CONTACT=0 in the environment selects the one-thread-per-particle search instead
   bool opt_contact_search = true;
@@ -177,6 +177,9 @@ struct mfg_particles {
   double2 *pse2 = nullptr;
   double *sums = nullptr;  // neighbour sums of the pair pass, PAIR_SUM_COUNT x sum_stride (SoA)
   size_t sum_stride = 0;
+  int *trav = nullptr;     // visiting order of the list pair pass (N_init)
+  int *seg_pos = nullptr;  // its segment offsets (lazily sized)
+  int seg_cap = 0;
   int *nbr = nullptr;
   int *count = nullptr;
   int nl_stride = 0;
@@ -279,7 +282,7 @@ size_t padded(size_t n) { return (n + 63) / 64 * 64; }
 
 void free_particles(mfg_particles *p) {
   void *ptrs[] = {p->arena[0], p->arena[1], p->uarena[0], p->uarena[1], p->rec, p->nl4, p->pse2,
-                  p->sums, p->nbr, p->count, p->key, p->key_sorted, p->order, p->blk, p->cnt, p->partials,
+                  p->sums, p->trav, p->seg_pos, p->nbr, p->count, p->key, p->key_sorted, p->order, p->blk, p->cnt, p->partials,
                   p->p_dbg, p->R_dbg, p->flag_d, p->stat_part, p->cs_list, p->cs_n,
                   p->cs_slot, p->cs_res};
   for (void *q : ptrs)
@@ -738,6 +741,7 @@ int mfg_particle_gpu_create(unsigned N, const double *pos4, const double *vel4, 
   if (hipMalloc(&p->rec, rec_bytes * n) != hipSuccess || hipMalloc(&p->nl4, sizeof(double4) * n) != hipSuccess ||
       hipMalloc(&p->pse2, sizeof(double2) * n) != hipSuccess ||
       hipMalloc(&p->sums, sizeof(double) * PAIR_SUM_COUNT * n) != hipSuccess ||
+      hipMalloc(&p->trav, sizeof(int) * n) != hipSuccess ||
       hipMalloc(&p->nbr, sizeof(int) * (size_t)p->cap * n) != hipSuccess ||
       hipMalloc(&p->count, sizeof(int) * n) != hipSuccess || hipMalloc(&p->key, sizeof(int) * n) != hipSuccess ||
       hipMalloc(&p->key_sorted, sizeof(int) * n) != hipSuccess ||
@@ -1527,12 +1531,35 @@ int mfg_leap_frog_step(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int rec
     if (rc != MFG_OK) return rc;
   }
   {
+    const int tiled = g_lib.opt.pair_pass == MFG_PAIR_TILED ? 1 : 0;
+    const GridInfo gi = g->info();
+    // strip width: 8 grid columns for 192-byte FP64 records, 16 for 112-byte FP32 records
+    // (measured best on the 1M single-grain cut); negative: the sorted order
+    const int strip = g_lib.opt.pair_strip > 0    ? g_lib.opt.pair_strip
+                      : g_lib.opt.pair_strip < 0  ? 0
+                      : p->precision == MFG_FP64 ? 8
+                                                 : 16;
+    const int *trav = nullptr;
+    if (!tiled && strip > 0) {
+      ProfScope ps("pair_order");
+      const int nseg = pair_order_segments(gi, strip);
+      if (nseg > p->seg_cap) {
+        if (p->seg_pos) (void)hipFree(p->seg_pos);
+        p->seg_pos = nullptr;
+        p->seg_cap = 0;
+        if (hipMalloc(&p->seg_pos, sizeof(int) * (size_t)nseg) != hipSuccess)
+          return fail(MFG_ERR_NOMEM, "pair-pass order (%d segments)", nseg);
+        p->seg_cap = nseg;
+      }
+      launch_pair_order(s, p->N_init, gi, g->count, p->key_sorted, p->cnt, strip, p->seg_pos,
+                        p->trav);
+      trav = p->trav;
+    }
     ProfScope ps(p->precision == MFG_FP64          ? "pair_fp64"
                  : p->precision == MFG_FP32_KAHAN ? "pair_fp32_kahan"
                                                   : "pair_fp32_wide");
-    launch_pair(s, p->N_init, p->st(), p->rec, p->nbr, p->nl_stride, p->count, c, p->cnt,
-                g->info(), g->count, p->key_sorted, g_lib.opt.pair_pass == MFG_PAIR_TILED ? 1 : 0, p->sums,
-                p->sum_stride);
+    launch_pair(s, p->N_init, p->st(), p->rec, p->nbr, p->nl_stride, p->count, c, p->cnt, gi,
+                g->count, p->key_sorted, tiled, p->sums, p->sum_stride, trav);
   }
   {
     ProfScope ps("particle_update");

@@ -845,19 +845,85 @@ struct Pair32 {
 };
 
 // ---------------------------------------------------------------------------------------------
+// Visiting order of the list-gather pair pass.  In the sorted order (x slowest, then y, then z)
+// a neighbour record is read again by the particles of the 5 x 5 grid columns around it, up to
+// four x-rows later; for a 1M-particle block that window holds about 7 MB of FP64 records, more
+// than the 4 MB L2 of an XCD, so records are fetched again from beyond the L2.  The pass
+// therefore visits the particles in strips of `strip` grid columns along y: strip by strip, and
+// inside a strip x-row by x-row.  Each (strip, x-row) segment is one contiguous range of the
+// sorted order, and the window shrinks to about 5 x (strip + 4) columns.  trav[t] is the
+// particle visited at position t < N.  The order changes which particle a thread takes, never
+// the sums of a particle.
+constexpr int TRAV_BLOCK = 1024;
+
+__device__ __forceinline__ int trav_segment_start(const GridInfo &g, const int *off, int strip,
+                                                  int q, int &size) {
+  const int nx = g.nx, s = q / nx, gi = q - s * nx;
+  const int j0 = s * strip, j1 = min(j0 + strip, g.ny);
+  const int a = off[((long)gi * g.ny + j0) * g.nz], b = off[((long)gi * g.ny + j1) * g.nz];
+  size = b - a;
+  return a;
+}
+
+// exclusive prefix of the segment sizes in visiting order (segment q = strip * nx + x-row);
+// one workgroup
+__global__ void __launch_bounds__(TRAV_BLOCK)
+    k_trav_segments(GridInfo g, const int *__restrict__ off, int strip, int nseg,
+                    int *__restrict__ seg_pos) {
+  __shared__ int s_part[TRAV_BLOCK];
+  const int per = (nseg + TRAV_BLOCK - 1) / TRAV_BLOCK;
+  const int beg = threadIdx.x * per, end = min(beg + per, nseg);
+  int sum = 0, size;
+  for (int q = beg; q < end; q++) {
+    trav_segment_start(g, off, strip, q, size);
+    sum += size;
+  }
+  s_part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int o = 1; o < TRAV_BLOCK; o <<= 1) {
+    const int v = threadIdx.x >= (unsigned)o ? s_part[threadIdx.x - o] : 0;
+    __syncthreads();
+    s_part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = s_part[threadIdx.x] - sum;
+  for (int q = beg; q < end; q++) {
+    trav_segment_start(g, off, strip, q, size);
+    seg_pos[q] = run;
+    run += size;
+  }
+}
+
+__global__ void __launch_bounds__(BSP)
+    k_trav_scatter(GridInfo g, const int *__restrict__ off, const int *__restrict__ key_sorted,
+                   const int *__restrict__ seg_pos, int strip, const Counters *cnt,
+                   int *__restrict__ trav) {
+  const unsigned p = blockIdx.x * BSP + threadIdx.x;
+  if (p >= cnt->N) return;
+  const int col = key_sorted[p] / g.nz;  // cell index of a particle in a cell
+  const int gi = col / g.ny, s = (col - gi * g.ny) / strip;
+  const int q = s * g.nx + gi;
+  int size;
+  const int a = trav_segment_start(g, off, strip, q, size);
+  trav[seg_pos[q] + ((int)p - a)] = (int)p;
+}
+
+// ---------------------------------------------------------------------------------------------
 // List-gather pair pass: one thread per particle, each neighbour record gathered from memory
-// where it is used (two record buffers in flight, gather_loop).  Kept beside the tiled pass as
-// the reference point of its measurements and for grids the tiled pass does not take.
+// where it is used (two record buffers in flight, gather_loop); thread t takes particle trav[t]
+// (t itself without a visiting order).
 template <class Body>
 __global__ void __launch_bounds__(BSP, PAIR64_WAVES)
     k_pair_list(unsigned n_init, State st, const typename Body::Rec *__restrict__ rec,
                 const int *__restrict__ nbr, int stride, const int *__restrict__ count, Consts c,
-                const Counters *cnt, double *__restrict__ sums, size_t sstride) {
+                const Counters *cnt, double *__restrict__ sums, size_t sstride,
+                const int *__restrict__ trav) {
   typedef typename Body::Rec Rec;
   typedef typename Body::CRec CRec;
-  const unsigned i = xcd_block() * BSP + threadIdx.x;
+  const unsigned t = xcd_block() * BSP + threadIdx.x;
   const unsigned N = cnt->N;
-  if (i >= N || i >= n_init) return;
+  if (t >= N || t >= n_init) return;
+  const unsigned i = trav ? (unsigned)trav[t] : t;
   const uint32_t fl = st.flags[i];
   if (fl & FL_BLANKED) return;
   Body P(c, rec[i], fl);
@@ -888,10 +954,12 @@ template <class Acc>
 __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
     k_pair32_coop(unsigned n_init, State st, const Rec32 *__restrict__ rec,
                   const int *__restrict__ nbr, int stride, const int *__restrict__ count,
-                  Consts c, const Counters *cnt, double *__restrict__ sums, size_t sstride) {
-  const unsigned i = xcd_block() * BSP + threadIdx.x;
+                  Consts c, const Counters *cnt, double *__restrict__ sums, size_t sstride,
+                  const int *__restrict__ trav) {
+  const unsigned t = xcd_block() * BSP + threadIdx.x;
   const unsigned N = cnt->N;
-  const bool in_range = i < N && i < n_init;
+  const bool in_range = t < N && t < n_init;
+  const unsigned i = in_range && trav ? (unsigned)trav[t] : t;
   const uint32_t fl = in_range ? st.flags[i] : FL_BLANKED;
   const bool act = in_range && !(fl & FL_BLANKED);
   const unsigned ic = act ? i : 0u;
@@ -1184,7 +1252,7 @@ void launch_neighbours(hipStream_t s, const GridInfo &g, const int *offsets, con
 void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *rec, const int *nbr,
                  int nl_stride, const int *count, const Consts &c, Counters *cnt,
                  const GridInfo &g, const int *offsets, const int *key_sorted, int tiled,
-                 double *sums, size_t sstride) {
+                 double *sums, size_t sstride, const int *trav) {
   if (tiled) {
     const dim3 grid((unsigned)((long)g.nx * g.ny)), block(64);
     if (c.precision == MFG_FP64)
@@ -1204,13 +1272,26 @@ void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *re
   const dim3 grid(nb(n_init, BSP)), block(BSP);
   if (c.precision == MFG_FP64)
     hipLaunchKernelGGL((k_pair_list<Pair64>), grid, block, 0, s, n_init, st, (const Rec64 *)rec,
-                       nbr, nl_stride, count, c, cnt, sums, sstride);
+                       nbr, nl_stride, count, c, cnt, sums, sstride, trav);
   else if (c.precision == MFG_FP32_KAHAN)
     hipLaunchKernelGGL((k_pair32_coop<AccKahan>), grid, block, 0, s, n_init, st,
-                       (const Rec32 *)rec, nbr, nl_stride, count, c, cnt, sums, sstride);
+                       (const Rec32 *)rec, nbr, nl_stride, count, c, cnt, sums, sstride, trav);
   else
     hipLaunchKernelGGL((k_pair32_coop<AccWide>), grid, block, 0, s, n_init, st,
-                       (const Rec32 *)rec, nbr, nl_stride, count, c, cnt, sums, sstride);
+                       (const Rec32 *)rec, nbr, nl_stride, count, c, cnt, sums, sstride, trav);
+}
+
+int pair_order_segments(const GridInfo &g, int strip) {
+  return (int)(((long)g.ny + strip - 1) / strip * g.nx);
+}
+
+void launch_pair_order(hipStream_t s, unsigned n_init, const GridInfo &g, const int *offsets,
+                       const int *key_sorted, const Counters *cnt, int strip, int *seg_pos,
+                       int *trav) {
+  hipLaunchKernelGGL(k_trav_segments, dim3(1), dim3(TRAV_BLOCK), 0, s, g, offsets, strip,
+                     pair_order_segments(g, strip), seg_pos);
+  hipLaunchKernelGGL(k_trav_scatter, dim3(nb(n_init, BSP)), dim3(BSP), 0, s, g, offsets,
+                     key_sorted, seg_pos, strip, cnt, trav);
 }
 
 void launch_particle_update(hipStream_t s, unsigned n_init, const State &st, const double *sums,

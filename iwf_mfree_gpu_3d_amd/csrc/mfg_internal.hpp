@@ -190,7 +190,14 @@ void launch_neighbours(hipStream_t s, const GridInfo &g, const int *offsets, con
 void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *rec, const int *nbr,
                  int nl_stride, const int *count, const Consts &c, Counters *cnt,
                  const GridInfo &g, const int *offsets, const int *key_sorted, int tiled,
-                 double *sums, size_t sum_stride);
+                 double *sums, size_t sum_stride, const int *trav);
+// visiting order of the list pair pass: strips of `strip` grid columns along y, x-row by x-row
+// inside a strip (trav[t] = particle visited at position t < N); seg_pos holds
+// pair_order_segments(g, strip) ints
+int pair_order_segments(const GridInfo &g, int strip);
+void launch_pair_order(hipStream_t s, unsigned n_init, const GridInfo &g, const int *offsets,
+                       const int *key_sorted, const Counters *cnt, int strip, int *seg_pos,
+                       int *trav);
 // neighbour sums per particle written by the pair pass (SoA, sum_stride apart)
 constexpr int PAIR_SUM_COUNT = 25;
 // Jaumann, continuity, momentum, advection, invalidation, leap-frog correction and Johnson-Cook

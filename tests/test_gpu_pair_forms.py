@@ -1,10 +1,13 @@
-"""Cell-tiled pair pass (MFG_PAIR_TILED) against the list-gather pass (MFG_PAIR_LIST).
+"""Forms of the pair pass against each other.
 
-Both passes form the same neighbour sums in the same order (every particle walks its own list
-from first to last entry), so FP64 results are bit-identical; in the FP32 modes only the FMA
-contraction the compiler picks in each kernel can differ, a rounding-level difference.
-The 1M-particle case puts more than 64 particles into every grid column of the workpiece, so the
-tiled pass splits columns into several chunks there.
+- The cell-tiled pass (MFG_PAIR_TILED) against the list-gather pass (MFG_PAIR_LIST): both form
+  the same neighbour sums in the same order (every particle walks its own list from first to
+  last entry), so FP64 results are bit-identical; in the FP32 modes only the FMA contraction the
+  compiler picks in each kernel can differ, a rounding-level difference.  The 1M-particle case
+  puts more than 64 particles into every grid column of the workpiece, so the tiled pass splits
+  columns into several chunks there.
+- The list pass's visiting order (strips of grid columns) against the sorted order: the order
+  only decides which thread takes which particle, so every precision is bit-identical.
 """
 import numpy as np
 import pytest
@@ -17,9 +20,9 @@ FIELDS = [_lib.F_POS, _lib.F_VEL, _lib.F_RHO, _lib.F_S, _lib.F_T, _lib.F_EPS_PL,
           _lib.F_VEL_T, _lib.F_RHO_T, _lib.F_S_T, _lib.F_T_T, _lib.F_FC]
 
 
-def run_case(preset, scale, seed, precision, pair_pass, steps):
+def run_case(preset, scale, seed, precision, pair_pass, steps, pair_strip=0):
     sc = scenes.Scene(preset, scale, seed)
-    parts, grid = scenes.setup_cut(sc, precision=precision, pair_pass=pair_pass)
+    parts, grid = scenes.setup_cut(sc, precision=precision, pair_pass=pair_pass, pair_strip=pair_strip)
     api.global_time_current = sc.params.dt
     lf = api.leap_frog(sc.n, grid.num_cell())
     scenes.run(lf, parts, grid, steps)
@@ -53,6 +56,18 @@ def test_tiled_matches_list_multichunk_columns(hip_device):
     """1M-particle cut (about 110 particles per grid column), one FP64 step: bit-identical."""
     a = run_case(2, 1.0, 1, _lib.MFG_FP64, api.MFG_PAIR_LIST, 1)
     b = run_case(2, 1.0, 1, _lib.MFG_FP64, api.MFG_PAIR_TILED, 1)
+    assert a["N"] == b["N"]
+    for f in FIELDS:
+        np.testing.assert_array_equal(b[f], a[f], err_msg=f"field {f}")
+
+
+@pytest.mark.parametrize("precision", [_lib.MFG_FP64, _lib.MFG_FP32_KAHAN, _lib.MFG_FP32_WIDE],
+                         ids=["fp64", "kahan", "wide"])
+@pytest.mark.parametrize("strip", [3, 16])
+def test_visiting_order_does_not_change_results(hip_device, precision, strip):
+    """Strips of 3 and 16 grid columns against the sorted order, thermal-contact grain, 10 steps."""
+    a = run_case(2, 0.2, 3, precision, api.MFG_PAIR_LIST, 10, pair_strip=-1)
+    b = run_case(2, 0.2, 3, precision, api.MFG_PAIR_LIST, 10, pair_strip=strip)
     assert a["N"] == b["N"]
     for f in FIELDS:
         np.testing.assert_array_equal(b[f], a[f], err_msg=f"field {f}")
