@@ -270,6 +270,23 @@ int mfg_mesh_read_ldynak(const char *path, int quads, int *n_verts, int *n_tris,
 int mfg_mesh_read_json(const char *path, int *n_verts, int *n_tris, double *verts3, int *tris3);
 const char *mfg_mesh_last_error(void);
 
+/* ---- warm starts (set-up time) ------------------------------------------------------------ */
+/* Particle positions from a legacy VTK file, as vtk_read_init_pos (src/vtk_reader.cpp:29-69):
+ * the POINTS lines of exactly three space-separated fields, up to the CELLS line.  Call once
+ * with xyz NULL for n, then with 3 * n doubles.  Host only. */
+int mfg_vtk_read_positions(const char *path, int *n, double *xyz);
+/* Steady-state temperature samples "x y z T" (one per line, lines of exactly four
+ * space-separated fields), as the single-grain set-up reads them
+ * (src/Benchmarks/benchmarks_single_grain.cu:281-301).  Two calls as above, 4 * n doubles. */
+int mfg_read_steady_temperatures(const char *path, int *n, double *xyzt);
+/* interp_temps (src/Benchmarks/interp_utils.cpp:65-99) on the GPU: for each of the n_out points
+ * (3 doubles each) values[i] = sum over the n_in samples (x, y, z, v; 4 doubles each), in sample
+ * order, of weights * W(|p_i - x_j|, h) * v_j, W the cubic spline with the reference's sqrtf
+ * site.  Host arrays in and out; the reference also rewrites the points' w component and dumps
+ * debug files, neither of which is done here. */
+int mfg_interp_temps(const double *samples4, int n_in, double weights, double h,
+                     const double *points3, int n_out, double *values);
+
 /* ---- workload builder (set-up time, host only) --------------------------------------------- */
 typedef struct {
   int nx, ny, nz;             /* workpiece lattice */

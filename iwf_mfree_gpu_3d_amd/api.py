@@ -541,3 +541,40 @@ def read_mesh_ldynak(path: str, quads: bool = False):
 def read_mesh_json(path: str):
     """(verts, tris) of a MeshLab JSON triangle mesh, json_read_triangles (src/json_reader.cpp:108-117)."""
     return _read_mesh(_L().mfg_mesh_read_json, path.encode())
+
+
+# ---- warm starts ----------------------------------------------------------------------------
+def _read_fields(call, path: str, width: int):
+    n = C.c_int(0)
+    rc = call(path.encode(), C.byref(n), None)
+    if rc != 0:
+        raise MfgError(rc, _L().mfg_mesh_last_error().decode())
+    out = np.zeros((n.value, width))
+    rc = call(path.encode(), C.byref(n), out.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc != 0:
+        raise MfgError(rc, _L().mfg_mesh_last_error().decode())
+    return out
+
+
+def read_vtk_positions(path: str) -> np.ndarray:
+    """(n, 3) positions of a legacy VTK particle file, vtk_read_init_pos (src/vtk_reader.cpp:29-69)."""
+    return _read_fields(_L().mfg_vtk_read_positions, path, 3)
+
+
+def read_steady_temperatures(path: str) -> np.ndarray:
+    """(n, 4) steady-state samples x y z T, as the single-grain set-up reads them
+    (src/Benchmarks/benchmarks_single_grain.cu:281-301)."""
+    return _read_fields(_L().mfg_read_steady_temperatures, path, 4)
+
+
+def interp_temps(samples: np.ndarray, weights: float, h: float, points: np.ndarray) -> np.ndarray:
+    """interp_temps (src/Benchmarks/interp_utils.cpp:65-99) on the GPU: for every point,
+    sum_j weights * W(|p - x_j|, h) * v_j over the samples (x, y, z, v) in order."""
+    _lib.init()
+    a = np.ascontiguousarray(samples, dtype=np.float64).reshape(-1, 4)
+    p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros(p.shape[0])
+    dp = C.POINTER(C.c_double)
+    check(_L().mfg_interp_temps(a.ctypes.data_as(dp), a.shape[0], float(weights), float(h),
+                                p.ctypes.data_as(dp), p.shape[0], out.ctypes.data_as(dp)))
+    return out

@@ -1657,4 +1657,38 @@ int mfg_profile_get(int k, char *name48, long *launches, double *total_ms) {
   return MFG_OK;
 }
 
+// ---- warm starts ----
+int mfg_interp_temps(const double *samples4, int n_in, double weights, double h,
+                     const double *points3, int n_out, double *values) {
+  NEED_LIB();
+  if (n_in < 0 || n_out < 0 || (n_in > 0 && !samples4) || (n_out > 0 && (!points3 || !values)))
+    return fail(MFG_ERR_ARG, "interp_temps: %d samples, %d points", n_in, n_out);
+  if (!(h > 0.)) return fail(MFG_ERR_ARG, "interp_temps: smoothing length %g", h);
+  if (n_out == 0) return MFG_OK;
+  double *d_in = nullptr, *d_pts = nullptr, *d_out = nullptr;
+  auto release = [&](int code) {
+    if (d_in) (void)hipFree(d_in);
+    if (d_pts) (void)hipFree(d_pts);
+    if (d_out) (void)hipFree(d_out);
+    return code;
+  };
+  if (hipMalloc(&d_in, sizeof(double) * 4 * (size_t)std::max(n_in, 1)) != hipSuccess ||
+      hipMalloc(&d_pts, sizeof(double) * 3 * (size_t)n_out) != hipSuccess ||
+      hipMalloc(&d_out, sizeof(double) * (size_t)n_out) != hipSuccess)
+    return release(fail(MFG_ERR_NOMEM, "interp_temps buffers (%d samples, %d points)", n_in, n_out));
+  hipStream_t s = g_lib.stream;
+  if ((n_in > 0 && hipMemcpyAsync(d_in, samples4, sizeof(double) * 4 * (size_t)n_in,
+                                  hipMemcpyHostToDevice, s) != hipSuccess) ||
+      hipMemcpyAsync(d_pts, points3, sizeof(double) * 3 * (size_t)n_out, hipMemcpyHostToDevice,
+                     s) != hipSuccess)
+    return release(fail(MFG_ERR_HIP, "interp_temps upload"));
+  launch_interp_temps(s, (const double4 *)d_in, n_in, weights, h, d_pts, n_out, d_out);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(values, d_out, sizeof(double) * (size_t)n_out, hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return release(fail(MFG_ERR_HIP, "interp_temps"));
+  return release(MFG_OK);
+}
+
 }  // extern "C"

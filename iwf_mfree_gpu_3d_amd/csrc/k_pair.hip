@@ -42,8 +42,6 @@ constexpr int BSP = 256;
 #ifndef PAIR32_DEEP
 #define PAIR32_DEEP 0
 #endif
-constexpr double PI_D = 3.14159265358979323846;
-constexpr double INV_PI_D = 0.31830988618379067154;
 
 // ---------------------------------------------------------------------------------------------
 // One wavefront per cell.  All particles of a cell share the 25 candidate ranges, so the wave

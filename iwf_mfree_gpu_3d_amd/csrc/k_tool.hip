@@ -798,6 +798,52 @@ void launch_force_reduce(hipStream_t s, const float *partials, int nblocks, int 
   hipLaunchKernelGGL(k_force_reduce, dim3(1), dim3(64), 0, s, partials, nblocks, n_tools, forces);
 }
 
+// ---------------------------------------------------------------------------------------------
+// Warm start: uniformly weighted SPH interpolation of a scalar carried by samples (x, y, z, v)
+// onto points (src/Benchmarks/interp_utils.cpp:65-99): v(p) = sum_j weights W(|p - x_j|, h) v_j
+// over all samples in sample order, W the cubic spline with the reference's sqrtf site
+// (interp_utils.cpp:19-63).  One thread per point, the samples staged through LDS in tiles; every
+// sample's term is added (zero outside the support), as the reference does.
+constexpr int INTERP_TILE = 256;
+
+__global__ void __launch_bounds__(INTERP_TILE)
+    k_interp_temps(const double4 *__restrict__ in, int n_in, double weights, double h,
+                   const double *__restrict__ pts3, int n_out, double *__restrict__ out) {
+  __shared__ double4 s_in[INTERP_TILE];
+  const int i = blockIdx.x * INTERP_TILE + threadIdx.x;
+  const bool act = i < n_out;
+  const double px = act ? pts3[3 * i] : 0., py = act ? pts3[3 * i + 1] : 0.,
+               pz = act ? pts3[3 * i + 2] : 0.;
+  const double h1 = 1 / h;
+  const double fourh2 = 4 * h * h;
+  const double fac = (INV_PI_D)*h1 * h1 * h1;
+  double v = 0.;
+  for (int t0 = 0; t0 < n_in; t0 += INTERP_TILE) {
+    const int j = t0 + (int)threadIdx.x;
+    if (j < n_in) s_in[threadIdx.x] = in[j];
+    __syncthreads();
+    const int m = min(INTERP_TILE, n_in - t0);
+    if (act) {
+      for (int k = 0; k < m; k++) {
+        const double4 q = s_in[k];
+        const double xij = px - q.x, yij = py - q.y, zij = pz - q.z;
+        const double rr2 = xij * xij + yij * yij + zij * zij;
+        double w = 0., wx, wy, wz;
+        if (!(rr2 >= fourh2 || rr2 < 1e-8)) cubic_spline64(xij, yij, zij, rr2, h1, fac, 1, w, wx, wy, wz);
+        v += weights * w * q.w;
+      }
+    }
+    __syncthreads();
+  }
+  if (act) out[i] = v;
+}
+
+void launch_interp_temps(hipStream_t s, const double4 *in, int n_in, double weights, double h,
+                         const double *pts3, int n_out, double *out) {
+  hipLaunchKernelGGL(k_interp_temps, dim3((n_out + INTERP_TILE - 1) / INTERP_TILE),
+                     dim3(INTERP_TILE), 0, s, in, n_in, weights, h, pts3, n_out, out);
+}
+
 int valid_check(hipStream_t s, const State &st, unsigned n, int *d_flag) {
   (void)hipMemsetAsync(d_flag, 0, sizeof(int), s);
   hipLaunchKernelGGL(k_check_valid, dim3(nbk(n)), dim3(BST), 0, s, st, n, d_flag);

@@ -1,5 +1,16 @@
-// mesh_io.cpp -- tool mesh readers (host, set-up time): the surface of an LS-DYNA keyword solid
-// mesh and MeshLab JSON triangle meshes, as indexed triangle meshes for mfg_tool_3d_gpu_create.
+// mesh_io.cpp -- input files of the set-up (host): the surface of an LS-DYNA keyword solid mesh
+// and MeshLab JSON triangle meshes (indexed triangle meshes for mfg_tool_3d_gpu_create), and the
+// warm-start inputs: particle positions from a legacy VTK file and steady-state temperature
+// samples.
+//
+//   VTK       vtk_read_init_pos (src/vtk_reader.cpp:29-69): the lines after the line holding
+//             "POINTS", up to the first line holding "CELLS", that split into exactly three
+//             space-separated fields (lines of a single space are skipped)
+//   steady    the "x y z T" lines of the steady-state temperature file read by the single-grain
+//             set-up (src/Benchmarks/benchmarks_single_grain.cu:281-301): every line that splits
+//             into exactly four space-separated fields
+//   Fields are split at every single space, as std::getline with a ' ' delimiter does: two
+//   spaces in a row make an empty field, a trailing space none.
 //
 //   LS-DYNA   ldynak_read_triangles_from_tetmesh / _quadmesh (src/ldynak_reader.cpp:308-434):
 //             *NODE lines "nid x y z [tc rc]", *ELEMENT_SOLID lines "eid pid n1 .. n8" (tets
@@ -189,6 +200,67 @@ int deliver(const Mesh &m, int *n_verts, int *n_tris, double *verts3, int *tris3
 
 thread_local std::string g_mesh_err;
 
+std::vector<std::string> split_spaces(const std::string &s) {
+  std::vector<std::string> out;
+  size_t a = 0;
+  while (a < s.size()) {
+    const size_t b = s.find(' ', a);
+    if (b == std::string::npos) {
+      out.push_back(s.substr(a));
+      break;
+    }
+    out.push_back(s.substr(a, b - a));
+    a = b + 1;
+  }
+  return out;
+}
+
+bool parse_double(const std::string &t, double &v) {
+  const char *p = t.c_str();
+  char *end = nullptr;
+  v = std::strtod(p, &end);
+  return end != p;
+}
+
+int read_fields(const char *path, bool vtk, int width, std::vector<double> &out,
+                std::string &err) {
+  std::ifstream in(path);
+  if (!in) {
+    err = std::string("cannot open ") + path;
+    return MFG_ERR_ARG;
+  }
+  std::string line;
+  bool sec = !vtk;
+  while (std::getline(in, line)) {
+    if (vtk) {
+      if (line.find("POINTS") != std::string::npos) {
+        sec = true;
+        continue;
+      }
+      if (line == " ") continue;
+      if (sec && line.find("CELLS") != std::string::npos) break;
+    }
+    if (!sec) continue;
+    const std::vector<std::string> tk = split_spaces(line);
+    if ((int)tk.size() != width) continue;
+    for (int k = 0; k < width; k++) {
+      double v;
+      if (!parse_double(tk[k], v)) {
+        err = "malformed number in line: " + line;
+        return MFG_ERR_ARG;
+      }
+      out.push_back(v);
+    }
+  }
+  return MFG_OK;
+}
+
+int deliver_fields(const std::vector<double> &v, int width, int *n, double *out) {
+  if (n) *n = (int)(v.size() / width);
+  if (out) std::memcpy(out, v.data(), sizeof(double) * v.size());
+  return MFG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -208,6 +280,22 @@ int mfg_mesh_read_json(const char *path, int *n_verts, int *n_tris, double *vert
   const int rc = read_json(path, m, g_mesh_err);
   if (rc != MFG_OK) return rc;
   return deliver(m, n_verts, n_tris, verts3, tris3);
+}
+
+int mfg_vtk_read_positions(const char *path, int *n, double *xyz) {
+  if (!path) return MFG_ERR_ARG;
+  std::vector<double> v;
+  const int rc = read_fields(path, true, 3, v, g_mesh_err);
+  if (rc != MFG_OK) return rc;
+  return deliver_fields(v, 3, n, xyz);
+}
+
+int mfg_read_steady_temperatures(const char *path, int *n, double *xyzt) {
+  if (!path) return MFG_ERR_ARG;
+  std::vector<double> v;
+  const int rc = read_fields(path, false, 4, v, g_mesh_err);
+  if (rc != MFG_OK) return rc;
+  return deliver_fields(v, 4, n, xyzt);
 }
 
 const char *mfg_mesh_last_error(void) { return g_mesh_err.c_str(); }

@@ -26,6 +26,9 @@ __device__ __forceinline__ unsigned xcd_block() {
 }
 
 
+constexpr double PI_D = 3.14159265358979323846;
+constexpr double INV_PI_D = 0.31830988618379067154;  // M_1_PI
+
 // Correctly rounded square roots.  HIP's __fsqrt_rn maps to the approximate hardware sqrt
 // unless OCML_BASIC_ROUNDED_OPERATIONS is defined; the builtins are IEEE-rounded under the
 // default -fhip-fp32-correctly-rounded-divide-sqrt.

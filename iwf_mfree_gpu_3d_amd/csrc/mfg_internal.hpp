@@ -238,4 +238,8 @@ void launch_flip_normals(hipStream_t s, Tri *tris, int n);
 
 int valid_check(hipStream_t s, const State &st, unsigned n, int *d_flag);
 
+// warm start: v(p) = sum_j weights W(|p - x_j|, h) v_j of samples (x, y, z, v) at n_out points
+void launch_interp_temps(hipStream_t s, const double4 *in, int n_in, double weights, double h,
+                         const double *pts3, int n_out, double *out);
+
 }  // namespace mfg

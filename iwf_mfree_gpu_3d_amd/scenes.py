@@ -103,3 +103,21 @@ def run(stepper: api.leap_frog, particles, grid, n_steps: int, record_forces: bo
         stepper.step(particles, grid, record_forces)
         api.global_time_step += 1
         api.global_time_current += api.global_time_dt
+
+
+def steady_tool_temperatures(samples: np.ndarray, steady: np.ndarray, dz: float, hdx: float,
+                             T_ref: float) -> np.ndarray:
+    """Warm-start temperatures of tool particles from a steady-state solution
+    (src/Benchmarks/benchmarks_single_grain.cu:303-373): the steady samples (x, y, z, T) are
+    shifted so that their bounding-box minimum meets the tool samples' minimum, interpolated onto
+    the tool samples with weights dx_s^3 and smoothing length max(1.5 dx_s, hdx dz), dx_s the
+    largest coordinate difference of the first two steady samples, and clamped below at T_ref.
+    The reference assigns each tool particle the value of the next sample in its list (the
+    increment of its iterator comes first); here every particle gets its own value."""
+    steady = np.asarray(steady, dtype=np.float64).reshape(-1, 4)
+    samples = np.asarray(samples, dtype=np.float64).reshape(-1, 3)
+    dx_s = float(np.max(np.abs(steady[0, :3] - steady[1, :3])))
+    shifted = steady.copy()
+    shifted[:, :3] = steady[:, :3] - steady[:, :3].min(axis=0) + samples.min(axis=0)
+    T = api.interp_temps(shifted, dx_s ** 3, max(1.5 * dx_s, hdx * dz), samples)
+    return np.maximum(T, T_ref)

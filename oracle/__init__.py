@@ -112,6 +112,7 @@ def _declare(L):
         "o_get_forces": (None, [_vp, _fp]),
         "o_neighbours": (None, [_vp, _ip, _ip, C.c_int]),
         "o_cubic_spline": (None, [_dp, _dp, C.c_double, C.c_int, _dp]),
+        "o_interp_temps": (None, [_dp, C.c_int, C.c_double, C.c_double, _dp, C.c_int, _dp]),
         "o_lapl_pse": (C.c_double, [_dp, _dp, C.c_double]),
         "o_sigma_yield": (C.c_double, [C.POINTER(Joco), C.c_double, C.c_double, C.c_double]),
         "o_opt_fun": (C.c_double, [C.POINTER(Joco), C.c_double, C.c_double, C.c_double, C.c_double,
@@ -152,6 +153,16 @@ def cubic_spline(pi, pj, h, single_sqrt=True):
     out = np.zeros(4)
     a, b = _keep(pi, pj)
     lib().o_cubic_spline(_d(a), _d(b), h, int(single_sqrt), _d(out))
+    return out
+
+
+def interp_temps(samples, weights, h, points):
+    """interp_temps (src/Benchmarks/interp_utils.cpp:65-99): per point, the sum over the samples
+    (x, y, z, v) in order of weights * W(|p - x_j|, h) * v_j."""
+    a = np.ascontiguousarray(samples, dtype=np.float64).reshape(-1, 4)
+    p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros(p.shape[0])
+    lib().o_interp_temps(_d(a), a.shape[0], float(weights), float(h), _d(p), p.shape[0], _d(out))
     return out
 
 

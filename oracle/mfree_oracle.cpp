@@ -1956,6 +1956,24 @@ void o_neighbours(void *p, int *counts, int *lists, int cap) {
   }
 }
 
+// ---- warm start ----
+// interp_temps (src/Benchmarks/interp_utils.cpp:65-99): for each point, the sum over the samples
+// in order of weights * W * v, W the spline of interp_utils.cpp:19-63 (the kernels.cuh form with
+// its sqrtf site).  The reference's debug dumps of the inputs and outputs are not restated.
+void o_interp_temps(const double *in4, int n_in, double weights, double h, const double *pts3,
+                    int n_out, double *out) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n_out; i++) {
+    const v3 p(pts3[3 * i], pts3[3 * i + 1], pts3[3 * i + 2]);
+    double T = 0.;
+    for (int j = 0; j < n_in; j++) {
+      const w4 w = cubic_spline(p, v3(in4[4 * j], in4[4 * j + 1], in4[4 * j + 2]), h, true);
+      T += weights * w.w * in4[4 * j + 3];
+    }
+    out[i] = T;
+  }
+}
+
 // ---- stand-alone device functions for closed-form tests ----
 void o_cubic_spline(const double *pi, const double *pj, double h, int single, double *out) {
   w4 w = cubic_spline(v3(pi[0], pi[1], pi[2]), v3(pj[0], pj[1], pj[2]), h, single != 0);
