@@ -180,6 +180,38 @@ def copy_probe(mib: int) -> None:
     print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 5}))
 
 
+def sweep(args) -> None:
+    """Served or batched form: one JSON line with the aggregate particle-steps/s as the value and
+    the p50 / p99 time to result beside it."""
+    from iwf_mfree_gpu_3d_amd import _lib, serve
+    _lib.init(0)
+    if args.batched > 0:
+        reqs = serve.sweep_requests(count=args.batched, batched=True, seed=args.seed, n_min=5e3, n_max=2e5,
+                                    steps=(300, 300), fp32_share=1.0)
+        workload = (f"batched: {args.batched} orthogonal-cut instances at t=0, N log-uniform in [5e3, 2e5] "
+                    f"(4:2:1 blocks), depth U[5,40] um, speed U[5,50] m/s, rake U[-30,10] deg, JC A/B/C x U(0.9,1.1), "
+                    f"FP32+Kahan, 300 steps each")
+    else:
+        reqs = serve.sweep_requests(rate=4.0, duration=args.served, seed=args.seed)
+        workload = (f"served: Poisson arrivals 4/s over {args.served:g} s, N log-uniform in [2e4, 2e6] (4:2:1 blocks), "
+                    f"70% FP32+Kahan / 30% FP64, 100-1000 steps, parameters as the batched form")
+    log(f"{len(reqs)} requests, {sum(r.lattice()[0] * r.lattice()[1] * r.lattice()[2] for r in reqs)} particles in all")
+    res = serve.serve(reqs, hbm_budget=args.hbm_budget_gb * 1e9, log=log)
+    log(f"p50 {res['p50_s']:.2f} s, p99 {res['p99_s']:.2f} s, {res['particle_steps_per_s']:.4e} particle-steps/s "
+        f"over {res['makespan_s']:.1f} s")
+    out = {
+        "metric": "particle-steps/s at 1M/8M particles, FP32 vs FP64; p99 sweep latency",
+        "value": res["particle_steps_per_s"], "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": int(sum(r.steps for r in reqs)), "warmup": 0, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "mixed" if args.served > 0 else "f32",
+        "data": "synthetic (seeded lattices + jitter, wedge tools)",
+        "config": {"workload": workload, "requests": len(reqs), "hbm_budget_gb": args.hbm_budget_gb},
+        "sweep": {"p50_time_to_result_s": res["p50_s"], "p99_time_to_result_s": res["p99_s"],
+                  "makespan_s": res["makespan_s"], "particle_steps": res["particle_steps"]},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,9 +227,19 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain copy of this many MiB (profiler counter calibration)")
+    ap.add_argument("--served", type=float, default=0.0,
+                    help="served form instead: Poisson arrivals (4 requests/s) over this many seconds of "
+                         "cutting-sweep requests, N log-uniform in [2e4, 2e6], 70%% FP32+Kahan, 100-1000 steps")
+    ap.add_argument("--batched", type=int, default=0,
+                    help="batched form instead: this many instances at time zero, N log-uniform in "
+                         "[5e3, 2e5], FP32+Kahan, 300 steps")
+    ap.add_argument("--hbm-budget-gb", type=float, default=250.0)
     args = ap.parse_args()
     if args.copy_probe_mib > 0:
         copy_probe(args.copy_probe_mib)
+        return
+    if args.served > 0 or args.batched > 0:
+        sweep(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "

@@ -133,6 +133,12 @@ const char *mfg_last_error(void);
 int mfg_synchronize(void);                       /* check_cuda_error, src/types.cpp:20-49 */
 void *mfg_get_stream(void);                      /* hipStream_t of the library */
 int mfg_set_options(const mfg_options *opt);     /* src/types.h:32-108 */
+/* Clears every constant set by the *_setup_* calls below, and the switches they turn on (thermal,
+ * friction, plasticity), to the state after mfg_init.  No counterpart in the reference, whose
+ * constants live as long as the process: it is for callers that run several set-ups in one
+ * process, such as a request loop serving many cutting instances, which re-installs an
+ * instance's constants before each of its steps. */
+int mfg_constants_reset(void);
 int mfg_get_options(mfg_options *opt);
 int mfg_version(void);
 

@@ -185,6 +185,7 @@ SIGNATURES = {
     "mfg_mesh_read_json": (_i, [C.c_char_p, _ip, _ip, _dp, _ip]),
     "mfg_mesh_last_error": (C.c_char_p, []),
     "mfg_vtk_read_positions": (_i, [C.c_char_p, _ip, _dp]),
+    "mfg_constants_reset": (_i, []),
     "mfg_read_steady_temperatures": (_i, [C.c_char_p, _ip, _dp]),
     "mfg_interp_temps": (_i, [_dp, _i, C.c_double, C.c_double, _dp, _i, _dp]),
 }

@@ -78,6 +78,12 @@ def make_joco_constants() -> joco_constants:
     return joco_constants()
 
 
+def constants_reset() -> None:
+    """Clears every constant and switch of the *_setup_* calls (see mfg_constants_reset)."""
+    _lib.init()
+    check(_L().mfg_constants_reset())
+
+
 MFG_PAIR_LIST, MFG_PAIR_TILED = 0, 1
 
 

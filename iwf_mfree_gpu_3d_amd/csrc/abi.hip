@@ -616,6 +616,15 @@ int mfg_synchronize(void) {
   return MFG_OK;
 }
 
+int mfg_constants_reset(void) {
+  g_lib.a_phys = g_lib.i_phys = mfg_phys_constants{};
+  g_lib.a_corr = g_lib.i_corr = mfg_corr_constants{};
+  g_lib.joco = mfg_joco_constants{};
+  g_lib.a_twp = g_lib.a_ttool = g_lib.i_twp = g_lib.i_ttool = mfg_trml_constants{};
+  g_lib.plastic = g_lib.a_thermal = g_lib.a_fric = g_lib.i_thermal_wp = false;
+  return MFG_OK;
+}
+
 int mfg_set_options(const mfg_options *o) {
   if (!o) return fail(MFG_ERR_ARG, "null options");
   if (o->heat < MFG_HEAT_NONE || o->heat > MFG_HEAT_PSE) return fail(MFG_ERR_ARG, "heat model %d", o->heat);

@@ -1,0 +1,98 @@
+"""Served and batched forms: the request schedule, admission by HBM budget, and (GPU) that a
+request's result is the same alone and in a mix of requests."""
+import numpy as np
+import pytest
+
+from iwf_mfree_gpu_3d_amd import _lib, api, serve
+
+
+def test_sweep_requests_follow_the_sweep():
+    reqs = serve.sweep_requests(rate=4.0, duration=100.0, seed=3)
+    assert 300 < len(reqs) < 500  # Poisson, mean 400
+    arr = np.array([r.arrival for r in reqs])
+    assert (np.diff(arr) > 0).all() and arr[-1] <= 100.0
+    n = np.array([r.n_target for r in reqs])
+    assert n.min() >= 2e4 and n.max() <= 2e6
+    # log-uniform: the median sits near the geometric mean of the range
+    assert 1e5 < np.median(n) < 4e5
+    fp32 = np.mean([r.precision == _lib.MFG_FP32_KAHAN for r in reqs])
+    assert 0.6 < fp32 < 0.8
+    assert all(100 <= r.steps <= 1000 for r in reqs)
+    assert all(5e-4 <= r.depth <= 4e-3 and 5e-4 <= r.speed <= 5e-3 and -30 <= r.rake_deg <= 10 for r in reqs)
+    assert all(0.9 <= f <= 1.1 for r in reqs for f in r.jc_scale)
+    for r in reqs[:20]:
+        nx, ny, nz = r.lattice()
+        assert nx == 2 * ny == 4 * nz and nx * ny * nz >= r.n_target
+    batch = serve.sweep_requests(count=16, batched=True, n_min=5e3, n_max=2e5, steps=(300, 300), fp32_share=1.0)
+    assert len(batch) == 16 and all(r.arrival == 0.0 and r.steps == 300 for r in batch)
+    assert all(r.precision == _lib.MFG_FP32_KAHAN for r in batch)
+
+
+class FakeInstance:
+    """Stands in for the device state: a request takes `steps` calls of advance(1)."""
+    active = 0
+    peak = 0
+
+    def __init__(self, req):
+        self.req, self.k, self.n = req, 0, 1000
+        FakeInstance.active += 1
+        FakeInstance.peak = max(FakeInstance.peak, FakeInstance.active)
+
+    def advance(self, steps):
+        k = min(steps, self.req.steps - self.k)
+        self.k += k
+        return k
+
+    @property
+    def done(self):
+        return self.k >= self.req.steps
+
+    def release(self):
+        FakeInstance.active -= 1
+
+
+def test_admission_respects_the_budget(monkeypatch):
+    monkeypatch.setattr(serve, "Instance", FakeInstance)
+    monkeypatch.setattr(_lib, "init", lambda *a, **k: None)
+    monkeypatch.setattr(api, "synchronize", lambda: None)
+    reqs = serve.sweep_requests(count=12, batched=True, n_min=1e5, n_max=1e5, steps=(20, 20), seed=2)
+    per = serve.device_bytes(reqs[0])
+    FakeInstance.active = FakeInstance.peak = 0
+    res = serve.serve(reqs, hbm_budget=2.5 * per, steps_per_round=5)
+    assert FakeInstance.peak == 2  # two fit the budget at a time
+    assert len(res["requests"]) == 12 and all("time_to_result" in r for r in res["requests"])
+    # first come, first served
+    admitted = [r["admitted"] for r in sorted(res["requests"], key=lambda r: r["rid"])]
+    assert admitted == sorted(admitted)
+    assert res["particle_steps"] == 12 * 20 * 1000
+    # a request larger than the whole budget still runs when it is alone
+    FakeInstance.active = FakeInstance.peak = 0
+    res = serve.serve(reqs[:2], hbm_budget=0.5 * per)
+    assert FakeInstance.peak == 1 and len(res["requests"]) == 2
+
+
+FIELDS = [_lib.F_POS, _lib.F_VEL, _lib.F_S, _lib.F_T, _lib.F_EPS_PL, _lib.F_RHO]
+
+
+@pytest.mark.gpu
+def test_result_same_alone_and_in_the_mix(hip_device):
+    """A request's fields after its last step are bit-identical whether it ran alone or
+    interleaved with others of different size, parameters and precision."""
+    reqs = serve.sweep_requests(count=3, batched=True, n_min=4e3, n_max=1.5e4, steps=(12, 12), seed=11,
+                                fp32_share=0.5)
+    reqs[1].precision = _lib.MFG_FP64 if reqs[0].precision != _lib.MFG_FP64 else _lib.MFG_FP32_KAHAN
+    reqs[2].arrival = 0.01
+
+    def grab(store):
+        def on_done(inst):
+            order = np.argsort(inst.particles.download(_lib.F_ID).astype(np.int64))
+            store[inst.req.rid] = {f: inst.particles.download(f)[order] for f in FIELDS}
+        return on_done
+
+    alone, mix = {}, {}
+    serve.serve([reqs[0]], on_done=grab(alone), steps_per_round=4)
+    res = serve.serve(reqs, on_done=grab(mix), steps_per_round=4)
+    assert sorted(mix) == [0, 1, 2] and len(res["requests"]) == 3
+    for f in FIELDS:
+        np.testing.assert_array_equal(mix[0][f], alone[0][f], err_msg=f"field {f}")
+    assert np.abs(alone[0][_lib.F_VEL]).max() > 0.0
