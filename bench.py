@@ -180,6 +180,59 @@ def copy_probe(mib: int) -> None:
     print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 5}))
 
 
+def error_report(args) -> None:
+    """The precision form: the FP32 modes against the FP64 run of the same workload after
+    --steps steps, particles matched by persistent id; relative L2 error
+    ||x - x_ref|| / ||x_ref|| and max absolute error of stress, plastic strain, temperature rise
+    and position."""
+    import numpy as np
+    from iwf_mfree_gpu_3d_amd import _lib, api, scenes
+    _lib.init(0)
+    fields = {"S": _lib.F_S, "eps_pl": _lib.F_EPS_PL, "T": _lib.F_T, "pos": _lib.F_POS}
+    runs, times = {}, {}
+    for name, prec in [("fp64", _lib.MFG_FP64), ("fp32-kahan", _lib.MFG_FP32_KAHAN),
+                       ("fp32-wide", _lib.MFG_FP32_WIDE)]:
+        scene = scenes.Scene(args.preset, args.scale, args.seed)
+        parts, grid = scenes.setup_cut(scene, precision=prec, single_sqrt=True)
+        lf = api.leap_frog(scene.n, grid.num_cell())
+        api.global_time_current = api.global_time_dt
+        api.synchronize()
+        t0 = time.perf_counter()
+        scenes.run(lf, parts, grid, args.steps)
+        api.synchronize()
+        times[name] = time.perf_counter() - t0
+        order = np.argsort(parts.download(_lib.F_ID).astype(np.int64))
+        runs[name] = {k: parts.download(f)[order] for k, f in fields.items()}
+        runs[name]["T"] = runs[name]["T"] - scene.params.trml_wp.T_init
+        log(f"{name}: {scene.n} particles, {args.steps} steps in {times[name]:.2f} s")
+        del parts, grid, lf
+    ref = runs["fp64"]
+    errors = {}
+    for name in ("fp32-kahan", "fp32-wide"):
+        e = {}
+        for k in fields:
+            a, b = runs[name][k], ref[k]
+            nb = np.linalg.norm(b.ravel())
+            e[k] = {"l2_rel": float(np.linalg.norm((a - b).ravel()) / nb) if nb > 0 else None,
+                    "max_abs": float(np.abs(a - b).max())}
+        errors[name] = e
+        log(f"{name} vs fp64: " + ", ".join(f"{k} L2 {v['l2_rel']:.2e} max {v['max_abs']:.2e}" for k, v in e.items()))
+    n = scene.n
+    out = {
+        "metric": "particle-steps/s at 1M/8M particles, FP32 vs FP64; p99 sweep latency",
+        "value": n * args.steps / times["fp32-kahan"], "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 0, "ms_per_step": times["fp32-kahan"] / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded lattice + jitter, synthetic grain meshes)",
+        "config": {"workload": WORKLOADS.get((args.preset, float(args.scale)), f"preset {args.preset}"),
+                   "particles": n, "precision": "fp32-kahan vs fp64"},
+        "particle_steps_per_s": {k: n * args.steps / v for k, v in times.items()},
+        "errors_vs_fp64": errors,
+        "error_note": "T is the temperature rise T - T_init; L2 relative = ||x - x_fp64|| / ||x_fp64||",
+    }
+    print(json.dumps(out))
+
+
 def sweep(args) -> None:
     """Served or batched form: one JSON line with the aggregate particle-steps/s as the value and
     the p50 / p99 time to result beside it."""
@@ -234,12 +287,18 @@ def main():
                     help="batched form instead: this many instances at time zero, N log-uniform in "
                          "[5e3, 2e5], FP32+Kahan, 300 steps")
     ap.add_argument("--hbm-budget-gb", type=float, default=250.0)
+    ap.add_argument("--error-report", action="store_true",
+                    help="precision form instead: run the workload --steps steps in FP64 and in both FP32 "
+                         "modes and report the FP32 L2 and max errors against the FP64 run")
     args = ap.parse_args()
     if args.copy_probe_mib > 0:
         copy_probe(args.copy_probe_mib)
         return
     if args.served > 0 or args.batched > 0:
         sweep(args)
+        return
+    if args.error_report:
+        error_report(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "

@@ -994,7 +994,18 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
   for (int k = 0; k < nmax; k++) {
     const bool more = k + 1 < nmax;
     if (more) CG::load((const char *)rec, sizeof(Rec32), idx_at(k + 1), lane, v);
-    if (k < nl) P.term(c, *(const Rec32 *)(s_rec[k & 1][w] + lane * sizeof(Rec32)));
+    if (k < nl) {
+      // the record as seven 16-byte reads: at the 112-byte slot stride the 8 lanes a 16-byte
+      // read serves per cycle fall on distinct banks, while 4- and 8-byte field reads would put
+      // 4 lanes on each bank
+      struct Pieces {
+        piece16 p[sizeof(Rec32) / 16];
+      } pc;
+      const char *src = s_rec[k & 1][w] + lane * sizeof(Rec32);
+#pragma unroll
+      for (int q = 0; q < (int)(sizeof(Rec32) / 16); q++) pc.p[q] = *(const piece16 *)(src + 16 * q);
+      P.term(c, __builtin_bit_cast(Rec32, pc));
+    }
     if (more) CG::store(s_rec[(k + 1) & 1][w], lane, v);
     wave_sync_lds();
   }
