@@ -180,7 +180,7 @@ class particle_gpu:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
+        if h and _L is not None:  # at interpreter exit the module may already be torn down
             _L().mfg_particle_gpu_destroy(h)
             self._h = None
 
@@ -237,7 +237,7 @@ class grid_base:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
+        if h and _L is not None:  # at interpreter exit the module may already be torn down
             _L().mfg_grid_destroy(h)
             self._h = None
 
@@ -333,7 +333,7 @@ class tool_3d_gpu:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
+        if h and _L is not None:  # at interpreter exit the module may already be torn down
             _L().mfg_tool_destroy(h)
             self._h = None
 
@@ -393,7 +393,7 @@ class tool_forces:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
+        if h and _L is not None:  # at interpreter exit the module may already be torn down
             _L().mfg_tool_forces_destroy(h)
             self._h = None
 
@@ -479,7 +479,7 @@ class leap_frog:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
+        if h and _L is not None:  # at interpreter exit the module may already be torn down
             _L().mfg_leap_frog_destroy(h)
             self._h = None
 

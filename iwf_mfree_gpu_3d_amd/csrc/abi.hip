@@ -178,6 +178,7 @@ struct mfg_particles {
   double *sums = nullptr;  // neighbour sums of the pair pass, PAIR_SUM_COUNT x sum_stride (SoA)
   size_t sum_stride = 0;
   int *trav = nullptr;     // visiting order of the list pair pass (N_init)
+  int *poison = nullptr;   // per particle: a non-finite record in its stencil (N_init)
   int *seg_pos = nullptr;  // its segment offsets (lazily sized)
   int seg_cap = 0;
   int *nbr = nullptr;
@@ -282,7 +283,7 @@ size_t padded(size_t n) { return (n + 63) / 64 * 64; }
 
 void free_particles(mfg_particles *p) {
   void *ptrs[] = {p->arena[0], p->arena[1], p->uarena[0], p->uarena[1], p->rec, p->nl4, p->pse2,
-                  p->sums, p->trav, p->seg_pos, p->nbr, p->count, p->key, p->key_sorted, p->order, p->blk, p->cnt, p->partials,
+                  p->sums, p->trav, p->seg_pos, p->poison, p->nbr, p->count, p->key, p->key_sorted, p->order, p->blk, p->cnt, p->partials,
                   p->p_dbg, p->R_dbg, p->flag_d, p->stat_part, p->cs_list, p->cs_n,
                   p->cs_slot, p->cs_res};
   for (void *q : ptrs)
@@ -436,7 +437,8 @@ int run_neighbours(mfg_grid *g, mfg_particles *p, const Consts &c, int want_pse)
   }
   p->stat_n = nb;
   launch_neighbours(g_lib.stream, gi, g->count, p->nl4, p->pse2, p->st().h, p->st().flags, c,
-                    p->nbr, p->nl_stride, p->count, p->st().T_t, p->cnt, want_pse, p->stat_part);
+                    p->nbr, p->nl_stride, p->count, p->st().T_t, p->cnt, want_pse, p->stat_part,
+                    p->poison);
   HIP_TRY(hipGetLastError());
   return MFG_OK;
 }
@@ -751,6 +753,7 @@ int mfg_particle_gpu_create(unsigned N, const double *pos4, const double *vel4, 
       hipMalloc(&p->pse2, sizeof(double2) * n) != hipSuccess ||
       hipMalloc(&p->sums, sizeof(double) * PAIR_SUM_COUNT * n) != hipSuccess ||
       hipMalloc(&p->trav, sizeof(int) * n) != hipSuccess ||
+      hipMalloc(&p->poison, sizeof(int) * n) != hipSuccess ||
       hipMalloc(&p->nbr, sizeof(int) * (size_t)p->cap * n) != hipSuccess ||
       hipMalloc(&p->count, sizeof(int) * n) != hipSuccess || hipMalloc(&p->key, sizeof(int) * n) != hipSuccess ||
       hipMalloc(&p->key_sorted, sizeof(int) * n) != hipSuccess ||
@@ -767,6 +770,7 @@ int mfg_particle_gpu_create(unsigned N, const double *pos4, const double *vel4, 
   (void)hipMemsetAsync(p->arena[1], 0, sizeof(double) * STATE_DOUBLES * n, s);
   (void)hipMemsetAsync(p->uarena[1], 0, sizeof(uint32_t) * 2 * n, s);
   (void)hipMemsetAsync(p->count, 0, sizeof(int) * n, s);
+  (void)hipMemsetAsync(p->poison, 0, sizeof(int) * n, s);
   (void)hipMemsetAsync(p->key_sorted, 0, sizeof(int) * n, s);
   (void)hipMemsetAsync(p->cnt, 0, sizeof(Counters), s);
   // host staging in the device layout
@@ -1572,7 +1576,7 @@ int mfg_leap_frog_step(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int rec
   }
   {
     ProfScope ps("particle_update");
-    launch_particle_update(s, p->N_init, p->st(), p->sums, p->sum_stride, c, p->cnt);
+    launch_particle_update(s, p->N_init, p->st(), p->sums, p->sum_stride, p->poison, c, p->cnt);
   }
   // tools
   ToolSet ts;

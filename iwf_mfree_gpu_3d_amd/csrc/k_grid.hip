@@ -334,7 +334,15 @@ __global__ void __launch_bounds__(BS)
   double vol = c.i_mass / rhoh;
   double cs = sqrt_site(c.i_K / rhoh, c.single_sqrt);
   store_rec<RecT>(rec, i, xh, h, vh, rhoh, A, R, Th, vol, cs, tool ? 1.0 : 0.0);
-  nl4[i] = make_double4(xh[0], xh[1], xh[2], blanked ? 1.0 : 0.0);
+  // candidate flag of the neighbour pass: 1 blanked, 2 a non-finite value among the fields a
+  // neighbour contributes (the reference's loops visit every candidate of the stencil, so
+  // 0 * NaN carries such a value into the sums of every particle around it)
+  bool finite = isfinite(vol) && isfinite(Th);
+#pragma unroll
+  for (int k = 0; k < 3; k++) finite = finite && isfinite(xh[k]) && isfinite(vh[k]);
+#pragma unroll
+  for (int k = 0; k < 6; k++) finite = finite && isfinite(A[k]) && isfinite(R[k]);
+  nl4[i] = make_double4(xh[0], xh[1], xh[2], blanked ? 1.0 : (finite ? 0.0 : 2.0));
   if (pse2) pse2[i] = make_double2(Th, c.i_mass / rhoh);  // (T, m / rho) of the PSE sum
   if (p_dbg) {
     p_dbg[i] = p;

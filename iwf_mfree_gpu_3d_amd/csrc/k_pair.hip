@@ -69,7 +69,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 struct NbParticle {
   double x, y, z, fourh2;
   double Ti, coef, inv_h2;
-  int blanked, pad;
+  int blanked, nonfinite;  // nonfinite: its own record holds a non-finite value (no list)
 };
 
 template <int PSE_MODE>
@@ -78,7 +78,7 @@ __global__ void __launch_bounds__(NB_BLOCK)
                       const double2 *__restrict__ pse2, const double *__restrict__ h,
                       const uint32_t *__restrict__ flags, Consts c, int *__restrict__ nbr, int cap,
                       int *__restrict__ count, double *__restrict__ T_t, Counters *cnt,
-                      unsigned long long *__restrict__ stat_part) {
+                      unsigned long long *__restrict__ stat_part, int *__restrict__ poison) {
   __shared__ int s_cand[NB_WAVES][NB_LCAP];
   __shared__ int s_pref[NB_WAVES][32];
   __shared__ int s_rst[NB_WAVES][32];
@@ -146,8 +146,8 @@ __global__ void __launch_bounds__(NB_BLOCK)
         NbParticle d;
         d.x = pi.x; d.y = pi.y; d.z = pi.z;
         d.fourh2 = 4 * hi * hi;
-        d.blanked = pi.w != 0. ? 1 : 0;
-        d.pad = 0;
+        d.blanked = pi.w == 1. ? 1 : 0;
+        d.nonfinite = pi.w == 2. ? 1 : 0;
         d.Ti = 0.; d.coef = 0.; d.inv_h2 = 0.;
         if (PSE_MODE) {
           d.Ti = pse2[p].x;
@@ -162,10 +162,12 @@ __global__ void __launch_bounds__(NB_BLOCK)
       wave_lds_sync();
       int np[NB_GROUP];
       double acc[NB_GROUP];
+      bool bad[NB_GROUP];
 #pragma unroll
       for (int u = 0; u < NB_GROUP; u++) {
         np[u] = 0;
         acc[u] = 0.;
+        bad[u] = false;
       }
       for (int b0 = 0; b0 < T; b0 += 64) {
         const int q = b0 + lane;
@@ -175,6 +177,7 @@ __global__ void __launch_bounds__(NB_BLOCK)
         double2 tr = make_double2(0., 1.);
         if (PSE_MODE) tr = pse2[j];
         const bool cand_ok = valid && pj.w == 0.;
+        const bool cand_bad = valid && pj.w == 2.;  // a non-finite neighbour record
 #pragma unroll
         for (int u = 0; u < NB_GROUP; u++) {
           if (u < gn) {
@@ -193,7 +196,8 @@ __global__ void __launch_bounds__(NB_BLOCK)
               const float w2 = (float)d.coef * __expf(-(float)rr2 * (float)d.inv_h2);
               acc[u] += (double)((float)(tr.x - d.Ti) * w2 * (float)tr.y);
             }
-            const bool ok = cand_ok && !d.blanked && inside;
+            bad[u] = bad[u] || cand_bad;
+            const bool ok = cand_ok && !d.blanked && !d.nonfinite && inside;
             const unsigned long long m = __ballot(ok);
             const int pos = np[u] + __popcll(m & ((1ull << lane) - 1ull));
             if (ok && pos < cap) nbr[(size_t)(g0 + u) * cap + pos] = j;
@@ -210,7 +214,9 @@ __global__ void __launch_bounds__(NB_BLOCK)
             for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
           }
           const int p = g0 + u;
+          const bool poisoned = __ballot(bad[u]) != 0;
           if (lane == 0) {
+            poison[p] = poisoned ? 1 : 0;
             if (np[u] > cap) {
               atomicOr(&cnt->err_capacity, 1);
               atomicMax(&cnt->max_count, np[u]);
@@ -500,7 +506,7 @@ __device__ __forceinline__ void store_sums(double *__restrict__ sums, size_t str
 // Johnson-Cook radial return of every particle the pair pass summed (not blanked, in a cell)
 __global__ void __launch_bounds__(BSP)
     k_particle_update(unsigned n_init, State st, const double *__restrict__ sums, size_t stride,
-                      Consts c, const Counters *cnt) {
+                      const int *__restrict__ poison, Consts c, const Counters *cnt) {
   const unsigned i = blockIdx.x * BSP + threadIdx.x;
   if (i >= cnt->N || i >= n_init) return;
   const uint32_t fl = st.flags[i];
@@ -519,6 +525,9 @@ __global__ void __launch_bounds__(BSP)
     ps.xt[a] = sums[(21 + a) * stride + i];
   }
   ps.Tl = sums[24 * stride + i];
+  // a non-finite record in the particle's stencil: the reference's sums carry it (0 * NaN), so
+  // its rates are non-finite and the invalidation blanks it
+  if (poison[i]) ps.vd.m[0][0] = __builtin_nan("");
   const bool tool_nz = (fl & FL_TOOL_NZ) != 0;
   const double thermal_alpha = tool_nz ? c.alpha_tool : c.alpha_wp;
   const bool brook = c.heat == MFG_HEAT_BROOKSHAW && thermal_alpha != 0.;
@@ -1247,11 +1256,11 @@ int neighbour_blocks(const GridInfo &g) { return (int)((g.num_cell + NB_WAVES - 
 void launch_neighbours(hipStream_t s, const GridInfo &g, const int *offsets, const double4 *nl4,
                        const double2 *pse2, const double *h, const uint32_t *flags,
                        const Consts &c, int *nbr, int cap, int *count, double *T_t, Counters *cnt,
-                       int want_pse, unsigned long long *stat_part) {
+                       int want_pse, unsigned long long *stat_part, int *poison) {
   const dim3 grid(neighbour_blocks(g)), block(NB_BLOCK);
 #define MFG_NB_LAUNCH(P)                                                                        \
   hipLaunchKernelGGL((k_cell_neighbours<P>), grid, block, 0, s, g, offsets, nl4, pse2, h, flags, c, \
-                     nbr, cap, count, T_t, cnt, stat_part)
+                     nbr, cap, count, T_t, cnt, stat_part, poison)
   if (!want_pse) MFG_NB_LAUNCH(0);
   else if (c.precision == MFG_FP64) MFG_NB_LAUNCH(1);
   else MFG_NB_LAUNCH(2);
@@ -1304,9 +1313,10 @@ void launch_pair_order(hipStream_t s, unsigned n_init, const GridInfo &g, const 
 }
 
 void launch_particle_update(hipStream_t s, unsigned n_init, const State &st, const double *sums,
-                            size_t sstride, const Consts &c, const Counters *cnt) {
+                            size_t sstride, const int *poison, const Consts &c,
+                            const Counters *cnt) {
   hipLaunchKernelGGL(k_particle_update, dim3(nb(n_init, BSP)), dim3(BSP), 0, s, n_init, st, sums,
-                     sstride, c, cnt);
+                     sstride, poison, c, cnt);
 }
 
 }  // namespace mfg

@@ -178,12 +178,13 @@ void launch_bbox_reduce(hipStream_t s, const State &st, const Counters *cnt, uns
                         double *partials, int nblocks);
 
 // neighbour lists (+ PSE heat) for every cell; stat_part gets 2 totals (candidates, pairs) per
-// workgroup, neighbour_blocks(g) workgroups
+// workgroup, neighbour_blocks(g) workgroups; poison[p] = 1 when a candidate of particle p holds a
+// non-finite record (nl4.w == 2), which the reference's sums carry into p's rates
 int neighbour_blocks(const GridInfo &g);
 void launch_neighbours(hipStream_t s, const GridInfo &g, const int *offsets, const double4 *nl4,
                        const double2 *pse2, const double *h, const uint32_t *flags,
                        const Consts &c, int *nbr, int cap, int *count, double *T_t, Counters *cnt,
-                       int want_pse, unsigned long long *stat_part);
+                       int want_pse, unsigned long long *stat_part, int *poison);
 // pair pass (neighbour sums).  tiled: one wavefront per grid column stages the neighbour
 // records of the column's particles in LDS (offsets and sorted keys of the grid); otherwise one
 // thread per particle gathers them from memory
@@ -203,7 +204,8 @@ constexpr int PAIR_SUM_COUNT = 25;
 // Jaumann, continuity, momentum, advection, invalidation, leap-frog correction and Johnson-Cook
 // radial return from the neighbour sums
 void launch_particle_update(hipStream_t s, unsigned n_init, const State &st, const double *sums,
-                            size_t sum_stride, const Consts &c, const Counters *cnt);
+                            size_t sum_stride, const int *poison, const Consts &c,
+                            const Counters *cnt);
 
 void launch_tool_update(hipStream_t s, Tri *tris, const double *p_init, int n, double t,
                         const double vel[3]);
