@@ -42,6 +42,10 @@ constexpr int BSP = 256;
 #ifndef PAIR32_DEEP
 #define PAIR32_DEEP 0
 #endif
+// 1: the FP64 list pass uses the cooperative gather through LDS (k_pair64_dma)
+#ifndef PAIR64_COOP
+#define PAIR64_COOP 1
+#endif
 
 // ---------------------------------------------------------------------------------------------
 // One wavefront per cell.  All particles of a cell share the 25 candidate ranges, so the wave
@@ -539,13 +543,13 @@ __global__ void __launch_bounds__(BSP)
 // (src/interactions_gpu.cu:152-471).  The list kernels and the tiled kernel below share it.
 struct Pair64 {
   typedef Rec64 Rec;
-  // what the CSPM sum reads of a neighbour: (x, y, z, h) and (T, vol, c, tool), the 16-byte
-  // pieces 0, 1, 10 and 11 of a record
+  // what the CSPM sum reads of a neighbour: (x, y, z, h) and (T, vol, c, tool), the first four
+  // 16-byte pieces of a record
   struct CRec {
     double4 p, q;
   };
   static constexpr int CPIECES = 4;
-  __device__ static __forceinline__ int cpiece(int pp) { return pp < 2 ? pp : pp + 8; }
+  __device__ static __forceinline__ int cpiece(int pp) { return pp; }
 
   Rec64 ri;
   bool tool_i, tool_nz, brook;
@@ -1022,6 +1026,193 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
   P.store(sums, sstride, i);
 }
 
+// FP64 list pass with a cooperative gather through LDS.  Gathered lane by lane, each 16-byte
+// piece of each neighbour record is its own L1 access, and the L1 serves about one access per
+// cycle: at 16 pieces per pair (CSPM sum and main sum) the per-lane pass is bound there.  Here
+// the wavefront loads the records of a step together, P pieces per record and 64 / P records
+// per load instruction, straight into LDS (global_load_lds_dwordx4: no VGPRs hold them in
+// flight), and every lane then reads its own record from LDS.  The CSPM sum walks the lists
+// with the records' first four pieces, the main sum with all twelve.  The sums of a particle
+// are those of k_pair_list<Pair64>: same terms, same order.
+//
+// LDS placement.  Load instruction q writes lane l's 16 bytes at DMA_BLOCK q + 16 l.  Each
+// group of four lanes (a quad) loads one 64-byte line of one record, so the L1 serves the quad
+// with one access; which line a quad takes and the order of the four pieces inside it are
+// chosen so that piece p of the records of any 16 consecutive lanes lies on 16 distinct 16-byte
+// bank groups (the lanes of one 16-byte LDS read phase).  With rot(L) = (L >> 2) & 3:
+//   CSPM walk (4 pieces, one line, 16 records per instruction): quad g of instruction q loads
+//   record 16 q + g, lane e of the quad its piece (e + rot) & 3.
+//   main walk (12 pieces, three lines, 5 records per instruction, 15 of 16 quads): line lam of
+//   record 5 q + rr goes to quad c + 4 k, c = (5 q + rr + lam) & 3 and k the rank of rr among
+//   the records of instruction q with a line of class c; lane e of the quad loads its piece
+//   4 lam + ((e + rot) & 3).
+// Piece p of lane L's record then sits at 64 quad + 16 ((p - rot) & 3) in block L's
+// instruction, on bank group 4 (quad & 3) + ((p - rot) & 3) = 4 ((L + lam) & 3) + ((p - rot) & 3).
+constexpr int DMA_BLOCK = 1280;  // 64 x 16 bytes, 256-byte aligned blocks
+struct CspmPlan {
+  static constexpr int P = 4, NI = 4;
+  unsigned *codes_;  // unused
+  int off_;          // this lane's piece offset (the same in every instruction)
+  __device__ void prepare(int lane) { off_ = 16 * (((lane & 3) + ((lane >> 4) & 3)) & 3); }
+  __device__ int src_lane(int q, int lane) const { return 16 * q + (lane >> 2); }
+  __device__ int off(int) const { return off_; }
+  __device__ static int read_pos(int L, int p) {
+    return DMA_BLOCK * (L >> 4) + 64 * (L & 15) + 16 * ((p - ((L >> 2) & 3)) & 3);
+  }
+};
+struct MainPlan {
+  static constexpr int P = 12, NI = 13;
+  // per instruction: source lane (bits 0-7) and piece offset (bits 8-15), two per word, kept in
+  // LDS (word m of lane l at codes + 4 (64 m + l)) to spare the registers
+  unsigned *codes_;
+  int lane_;
+  __device__ static int quad(int q, int rr, int lam) {
+    const int c = (5 * q + rr + lam) & 3;
+    const int rb = (c - 5 * q + 1) & 3;  // the one rr (mod 4) with no line of class c
+    return c + 4 * (rr - (rr > rb ? 1 : 0));
+  }
+  __device__ void prepare(int lane) {
+    lane_ = lane;
+    const int g = lane >> 2, e = lane & 3, c = g & 3, k = g >> 2;
+    unsigned word = 0;
+#pragma unroll
+    for (int q = 0; q < NI; q++) {
+      const int rb = (c - 5 * q + 1) & 3;
+      const int rr = k + (k >= rb ? 1 : 0);
+      int r = 5 * q + rr;
+      const bool on = k < (rb == 0 ? 3 : 4) && r < 64;
+      const int lam = (c - 5 * q - rr) & 3;
+      int pc = 4 * lam + ((e + ((r >> 2) & 3)) & 3);
+      if (!on) {  // an unused quad: reload piece 0 of the instruction's first record
+        r = 5 * q;
+        pc = 0;
+      }
+      const unsigned code = (unsigned)r | ((unsigned)(16 * pc) << 8);
+      if (q & 1) {
+        codes_[64 * (q >> 1) + lane] = word | code << 16;
+      } else {
+        word = code;
+        if (q == NI - 1) codes_[64 * (q >> 1) + lane] = word;
+      }
+    }
+  }
+  __device__ unsigned code(int q) const { return codes_[64 * (q >> 1) + lane_] >> (16 * (q & 1)); }
+  __device__ int src_lane(int q, int) const { return code(q) & 0xff; }
+  __device__ int off(int q) const { return (code(q) >> 8) & 0xff; }
+  __device__ static int read_pos(int L, int p) {
+    const int q = L / 5, rr = L - 5 * q;
+    return DMA_BLOCK * q + 64 * quad(q, rr, p >> 2) + 16 * ((p - ((L >> 2) & 3)) & 3);
+  }
+};
+constexpr int DMA64_WINDOWS = MainPlan::NI * DMA_BLOCK;  // the index windows follow the blocks
+constexpr int DMA64_CODES = DMA64_WINDOWS + 64 * 16;     // then the main walk's load codes
+constexpr int DMA64_AREA = DMA64_CODES + 64 * 4 * ((MainPlan::NI + 1) / 2);
+#ifndef PAIR64D_WAVES
+#define PAIR64D_WAVES 2
+#endif
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Walks the lists of a wavefront: step k gives each lane with k < nl the first P pieces of its
+// neighbour k.  Step k + 1's records and the index window of steps k + 2 .. k + 5 are loaded
+// while step k's term runs.
+template <class Plan, class Term>
+__device__ __forceinline__ void dma_list_walk(const Rec64 *__restrict__ rec, const int *row,
+                                              int nl, int nmax, unsigned ic, int lane,
+                                              char *area, Term &&term) {
+  constexpr int P = Plan::P, NI = Plan::NI;
+  static_assert(NI * DMA_BLOCK <= DMA64_WINDOWS, "area");
+  typedef int iv4 __attribute__((ext_vector_type(4)));
+  char *wslot = area + DMA64_WINDOWS;
+  Plan plan;
+  plan.codes_ = (unsigned *)(area + DMA64_CODES);
+  plan.prepare(lane);
+  iv4 wc = {0, 0, 0, 0};
+  if (nl > 0) wc = *(const iv4 *)row;
+  auto idx_at = [&](int k) -> int {
+    if (k >= nl) return (int)ic;  // past the end of the list: the lane's own record, unused
+    const int q = k & 3;
+    return q == 0 ? wc.x : q == 1 ? wc.y : q == 2 ? wc.z : wc.w;
+  };
+  auto issue = [&](int k) {
+    const int j = idx_at(k);
+    int jm[NI];
+#pragma unroll
+    for (int q = 0; q < NI; q++) jm[q] = __shfl(j, plan.src_lane(q, lane), 64);
+#pragma unroll
+    for (int q = 0; q < NI; q++) {
+      const char *src = (const char *)rec + (size_t)jm[q] * sizeof(Rec64) + plan.off(q);
+      __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)(area + DMA_BLOCK * q), 16,
+                                       0, 0);
+    }
+  };
+  if (nmax > 0) issue(0);
+  for (int k = 0; k < nmax; k++) {
+    __builtin_amdgcn_s_waitcnt(0);  // this step's records (and a window) are in LDS
+    struct Pieces {
+      piece16 p[P];
+    } pc;
+#pragma unroll
+    for (int p = 0; p < P; p++) pc.p[p] = *(const piece16 *)(area + Plan::read_pos(lane, p));
+    const bool next_window = ((k + 1) & 3) == 0 && k + 1 < nmax;
+    iv4 wl = wc;
+    if (next_window) wl = *(const iv4 *)(wslot + 16 * lane);
+    __builtin_amdgcn_s_waitcnt(0);  // every lane's reads are done before the area is refilled
+    wave_sync_lds();
+    if (k + 1 < nmax) {
+      wc = wl;
+      issue(k + 1);
+      // the window of steps k+2 .. k+5, read at step k+1 (a lane past its list reads its first)
+      if (((k + 2) & 3) == 0 && k + 2 < nmax) {
+        const int *src = k + 2 < nl ? row + k + 2 : row;
+        __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)wslot, 16, 0, 0);
+      }
+    }
+    if (k < nl) term(pc);
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  wave_sync_lds();  // the area is free for the next walk
+}
+
+__global__ void __launch_bounds__(BSP, PAIR64D_WAVES)
+    k_pair64_dma(unsigned n_init, State st, const Rec64 *__restrict__ rec,
+                 const int *__restrict__ nbr, int stride, const int *__restrict__ count,
+                 Consts c, const Counters *cnt, double *__restrict__ sums, size_t sstride,
+                 const int *__restrict__ trav) {
+  typedef Pair64::CRec CRec;
+  static_assert(sizeof(Rec64) == 192 && sizeof(CRec) == 64, "record pieces");
+  const unsigned t = xcd_block() * BSP + threadIdx.x;
+  const unsigned N = cnt->N;
+  const bool in_range = t < N && t < n_init;
+  const unsigned i = in_range && trav ? (unsigned)trav[t] : t;
+  const uint32_t fl = in_range ? st.flags[i] : FL_BLANKED;
+  const bool act = in_range && !(fl & FL_BLANKED);
+  const unsigned ic = act ? i : 0u;
+  Pair64 P(c, rec[ic], fl);
+  const int nl = act ? count[i] : 0;
+  const int *row = nbr + (size_t)ic * stride;
+  __shared__ __align__(16) char s_area[BSP / 64][DMA64_AREA];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  char *area = s_area[w];
+  if (c.cspm) {
+    // every lane of the wavefront takes part in the loads; a tool particle (no CSPM sum) walks
+    // an empty list
+    const int nc = P.cspm_sum() ? nl : 0;
+    int ncmax = nc;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) ncmax = max(ncmax, __shfl_xor(ncmax, o, 64));
+    dma_list_walk<CspmPlan>(rec, row, nc, ncmax, ic, lane, area,
+                     [&](const auto &pc) { P.cspm_term(c, __builtin_bit_cast(CRec, pc)); });
+    P.cspm_end();
+  }
+  int nmax = nl;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o, 64));
+  dma_list_walk<MainPlan>(rec, row, nl, nmax, ic, lane, area,
+                    [&](const auto &pc) { P.term(c, __builtin_bit_cast(Rec64, pc)); });
+  if (!act) return;
+  P.store(sums, sstride, i);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Cell-tiled pair pass.  One wavefront per grid column (gi, gj); the column's particles are
 // taken in chunks of at most 64 consecutive particles, one per lane.  Every neighbour of a chunk
@@ -1288,7 +1479,10 @@ void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *re
     return;
   }
   const dim3 grid(nb(n_init, BSP)), block(BSP);
-  if (c.precision == MFG_FP64)
+  if (c.precision == MFG_FP64 && PAIR64_COOP)
+    hipLaunchKernelGGL(k_pair64_dma, grid, block, 0, s, n_init, st, (const Rec64 *)rec, nbr,
+                       nl_stride, count, c, cnt, sums, sstride, trav);
+  else if (c.precision == MFG_FP64)
     hipLaunchKernelGGL((k_pair_list<Pair64>), grid, block, 0, s, n_init, st, (const Rec64 *)rec,
                        nbr, nl_stride, count, c, cnt, sums, sstride, trav);
   else if (c.precision == MFG_FP32_KAHAN)

@@ -56,10 +56,10 @@ constexpr int STATE_DOUBLES = 3 + 3 + 3 + 6 + 2 + 6 + 3 + 3 + 3 + 2 + 6;
 // pair record, FP64: 24 doubles
 struct alignas(16) Rec64 {
   double x, y, z, h;
+  double T, vol, c, tool;  // with x .. h: what the CSPM sum reads, the record's first 64 bytes
   double vx, vy, vz, rho;
   double A[6];  // (S - p I) / rho^2 for the diagonal, S / rho^2 off the diagonal
   double R[6];  // artificial stress
-  double T, vol, c, tool;
 };
 
 // pair record, FP32 pair arithmetic: positions stay FP64; the first 32 bytes hold everything
