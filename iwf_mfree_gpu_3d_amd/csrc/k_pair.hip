@@ -56,7 +56,13 @@ constexpr int BSP = 256;
 // independent work to overlap.  For each particle an ordered ballot compaction appends the kept
 // candidates to its row-major list; its PSE sum is a per-lane partial reduced over the wave.
 //   PSE_MODE 0: no PSE sum; 1: FP64 PSE terms; 2: FP32 PSE terms accumulated in FP64
-constexpr int NB_WAVES = 4;             // cells (wavefronts) per workgroup
+// Cells (wavefronts) per workgroup.  One: cells differ in particles and candidates, and the
+// waves of a larger workgroup wait for its slowest cell at the end (the statistics sum) while
+// holding their slots; measured 12-14 % slower with four.
+#ifndef NB_WAVES_PER_BLOCK
+#define NB_WAVES_PER_BLOCK 1
+#endif
+constexpr int NB_WAVES = NB_WAVES_PER_BLOCK;
 constexpr int NB_BLOCK = 64 * NB_WAVES;
 constexpr int NB_LCAP = 1024;           // candidates listed in LDS per cell (beyond: searched)
 #ifndef NB_GROUP_SIZE
