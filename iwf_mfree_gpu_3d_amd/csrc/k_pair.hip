@@ -82,6 +82,163 @@ struct NbParticle {
   int blanked, nonfinite;  // nonfinite: its own record holds a non-finite value (no list)
 };
 
+// Candidate q of a cell's list as the neighbour pass needs it: particle index, (x, y, z, flag)
+// and (T, m / rho).
+struct NbCand {
+  int j;
+  double4 pj;
+  double2 tr;
+};
+
+// The particles [s, e) of one cell against its T candidates (fetch(q, valid, g0) gives candidate
+// q; lanes with !valid get any readable record, masked off): support test, PSE sum, ordered list
+// write.  pd: this wavefront's LDS slots for a group's particle data.
+template <int PSE_MODE, class Fetch>
+__device__ __forceinline__ void cell_lists(int s, int e, int T, int lane, NbParticle *pd,
+                                           Fetch &&fetch, const double4 *__restrict__ nl4,
+                                           const double2 *__restrict__ pse2,
+                                           const double *__restrict__ h,
+                                           const uint32_t *__restrict__ flags, const Consts &c,
+                                           int *__restrict__ nbr, int cap, int *__restrict__ count,
+                                           double *__restrict__ T_t, Counters *cnt,
+                                           int *__restrict__ poison, unsigned long long &n_cand,
+                                           unsigned long long &n_pair) {
+  for (int g0 = s; g0 < e; g0 += NB_GROUP) {
+    const int gn = min(NB_GROUP, e - g0);
+    if (lane < gn) {
+      const int p = g0 + lane;
+      const double4 pi = nl4[p];
+      const double hi = h[p];
+      NbParticle d;
+      d.x = pi.x; d.y = pi.y; d.z = pi.z;
+      d.fourh2 = 4 * hi * hi;
+      d.blanked = pi.w == 1. ? 1 : 0;
+      d.nonfinite = pi.w == 2. ? 1 : 0;
+      d.Ti = 0.; d.coef = 0.; d.inv_h2 = 0.;
+      if (PSE_MODE) {
+        d.Ti = pse2[p].x;
+        const double h2 = hi * hi;
+        const double h4 = h2 * h2;
+        const double h5 = hi * h4;
+        d.coef = +4. / (h5 * PI_D * sqrt_rn(PI_D));
+        d.inv_h2 = 1. / h2;
+      }
+      pd[lane] = d;
+    }
+    wave_lds_sync();
+    int np[NB_GROUP];
+    double acc[NB_GROUP];
+    bool bad[NB_GROUP];
+#pragma unroll
+    for (int u = 0; u < NB_GROUP; u++) {
+      np[u] = 0;
+      acc[u] = 0.;
+      bad[u] = false;
+    }
+    for (int b0 = 0; b0 < T; b0 += 64) {
+      const int q = b0 + lane;
+      const bool valid = q < T;
+      const NbCand cd = fetch(q, valid, g0);
+      const int j = cd.j;
+      const double4 pj = cd.pj;
+      const double2 tr = cd.tr;
+      const bool cand_ok = valid && pj.w == 0.;
+      const bool cand_bad = valid && pj.w == 2.;  // a non-finite neighbour record
+#pragma unroll
+      for (int u = 0; u < NB_GROUP; u++) {
+        if (u < gn) {
+          const NbParticle &d = pd[u];
+          const double xij = d.x - pj.x, yij = d.y - pj.y, zij = d.z - pj.z;
+          const double rr2 = xij * xij + yij * yij + zij * zij;
+          const bool inside = !(rr2 >= d.fourh2 || rr2 < 1e-8);  // src/kernels.cuh:41
+          // lapl_pse (src/kernels.cuh:71-92) times (T_j - T_i) m / rho_j
+          // (src/interactions_gpu.cu:124-147); r^2 / h^2 as r^2 times 1 / h^2 and m / rho_j
+          // from the record, each within two ulps of the reference's sqrt-square and divisions
+          if (PSE_MODE == 1 && valid) {
+            const double w2 = d.coef * exp(-rr2 * d.inv_h2);
+            acc[u] += (tr.x - d.Ti) * w2 * tr.y;
+          }
+          if (PSE_MODE == 2 && valid) {
+            const float w2 = (float)d.coef * __expf(-(float)rr2 * (float)d.inv_h2);
+            acc[u] += (double)((float)(tr.x - d.Ti) * w2 * (float)tr.y);
+          }
+          bad[u] = bad[u] || cand_bad;
+          const bool ok = cand_ok && !d.blanked && !d.nonfinite && inside;
+          const unsigned long long m = __ballot(ok);
+          const int pos = np[u] + __popcll(m & ((1ull << lane) - 1ull));
+          if (ok && pos < cap) nbr[(size_t)(g0 + u) * cap + pos] = j;
+          np[u] += __popcll(m);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < NB_GROUP; u++) {
+      if (u < gn) {
+        double a = acc[u];
+        if (PSE_MODE) {
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+        }
+        const int p = g0 + u;
+        const bool poisoned = __ballot(bad[u]) != 0;
+        if (lane == 0) {
+          poison[p] = poisoned ? 1 : 0;
+          if (np[u] > cap) {
+            atomicOr(&cnt->err_capacity, 1);
+            atomicMax(&cnt->max_count, np[u]);
+          }
+          count[p] = np[u] > cap ? cap : np[u];
+          if (PSE_MODE) T_t[p] = ((flags[p] & FL_TOOL) ? c.alpha_tool : c.alpha_wp) * a;
+        }
+        n_cand += (unsigned long long)T;
+        n_pair += (unsigned long long)(np[u] > cap ? cap : np[u]);
+      }
+    }
+    wave_lds_sync();  // the next group overwrites pd
+  }
+}
+
+// The candidate ranges of one cell: for stencil column r = lane < nr, the particles of rows
+// [lo_k, hi_k) of grid column (lo_i + r / nj, lo_j + r % nj); T, and the exclusive prefix of the
+// range lengths and each range's first particle in LDS (pref, rst; entries past nr hold T).
+__device__ __forceinline__ int cell_ranges(const GridInfo &g, const int *__restrict__ off, int lane,
+                                           int lo_i, int lo_j, int nj, int nr, int lo_k, int hi_k,
+                                           int *pref, int *rst, int &st, int &len) {
+  const int nyz = g.ny * g.nz;
+  st = 0;
+  len = 0;
+  if (lane < nr) {
+    const int ii = lo_i + lane / nj, jj = lo_j + lane % nj;
+    const long cb = (long)ii * nyz + (long)jj * g.nz;
+    st = off[cb + lo_k];
+    len = off[cb + hi_k] - st;
+  }
+  int incl = len;
+#pragma unroll
+  for (int o = 1; o < 32; o <<= 1) {
+    const int t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  const int T = __builtin_amdgcn_readfirstlane(__shfl(incl, 31, 64));
+  if (lane < 32) {
+    pref[lane] = incl - len;
+    rst[lane] = st;
+  }
+  wave_lds_sync();
+  return T;
+}
+
+// last stencil column whose prefix is <= q (never an empty one)
+__device__ __forceinline__ int range_of(const int *pref, int nr, int q) {
+  int lo = 0, hi = nr - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pref[mid] <= q) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 template <int PSE_MODE>
 __global__ void __launch_bounds__(NB_BLOCK)
     k_cell_neighbours(GridInfo g, const int *__restrict__ off, const double4 *__restrict__ nl4,
@@ -109,26 +266,9 @@ __global__ void __launch_bounds__(NB_BLOCK)
     const int lo_j = gj - 2 < 0 ? 0 : gj - 2, hi_j = gj + 3 > g.ny ? g.ny : gj + 3;
     const int lo_k = gk - 2 < 0 ? 0 : gk - 2, hi_k = gk + 3 > g.nz ? g.nz : gk + 3;
     const int nj = hi_j - lo_j, nr = (hi_i - lo_i) * nj;
-    // candidate range of stencil column (ii, jj) = lane, its exclusive prefix over the columns
-    int st = 0, len = 0;
-    if (lane < nr) {
-      const int ii = lo_i + lane / nj, jj = lo_j + lane % nj;
-      const long cb = (long)ii * nyz + (long)jj * g.nz;
-      st = off[cb + lo_k];
-      len = off[cb + hi_k] - st;
-    }
-    int incl = len;
-#pragma unroll
-    for (int o = 1; o < 32; o <<= 1) {
-      const int t = __shfl_up(incl, o, 64);
-      if (lane >= o) incl += t;
-    }
-    const int T = __builtin_amdgcn_readfirstlane(__shfl(incl, 31, 64));
-    if (lane < 32) {
-      s_pref[w][lane] = incl - len;  // columns >= nr: T
-      s_rst[w][lane] = st;
-    }
-    wave_lds_sync();
+    int st, len;
+    const int T = cell_ranges(g, off, lane, lo_i, lo_j, nj, nr, lo_k, hi_k, s_pref[w], s_rst[w],
+                              st, len);
     const bool in_lds = T <= NB_LCAP;
     if (in_lds) {
       for (int r = 0; r < nr; r++) {
@@ -137,109 +277,25 @@ __global__ void __launch_bounds__(NB_BLOCK)
       }
       wave_lds_sync();
     }
-    auto cand_at = [&](int q) -> int {
-      if (in_lds) return s_cand[w][q];
-      int lo = 0, hi = nr - 1;  // last column whose prefix is <= q (never an empty one)
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (s_pref[w][mid] <= q) lo = mid;
-        else hi = mid - 1;
+    auto fetch = [&](int q, bool valid, int g0) {
+      NbCand cd;
+      int j = g0;
+      if (valid) {
+        if (in_lds) {
+          j = s_cand[w][q];
+        } else {
+          const int r = range_of(s_pref[w], nr, q);
+          j = s_rst[w][r] + (q - s_pref[w][r]);
+        }
       }
-      return s_rst[w][lo] + (q - s_pref[w][lo]);
+      cd.j = j;
+      cd.pj = nl4[j];
+      cd.tr = make_double2(0., 1.);
+      if (PSE_MODE) cd.tr = pse2[j];
+      return cd;
     };
-    for (int g0 = s; g0 < e; g0 += NB_GROUP) {
-      const int gn = min(NB_GROUP, e - g0);
-      if (lane < gn) {
-        const int p = g0 + lane;
-        const double4 pi = nl4[p];
-        const double hi = h[p];
-        NbParticle d;
-        d.x = pi.x; d.y = pi.y; d.z = pi.z;
-        d.fourh2 = 4 * hi * hi;
-        d.blanked = pi.w == 1. ? 1 : 0;
-        d.nonfinite = pi.w == 2. ? 1 : 0;
-        d.Ti = 0.; d.coef = 0.; d.inv_h2 = 0.;
-        if (PSE_MODE) {
-          d.Ti = pse2[p].x;
-          const double h2 = hi * hi;
-          const double h4 = h2 * h2;
-          const double h5 = hi * h4;
-          d.coef = +4. / (h5 * PI_D * sqrt_rn(PI_D));
-          d.inv_h2 = 1. / h2;
-        }
-        s_pd[w][lane] = d;
-      }
-      wave_lds_sync();
-      int np[NB_GROUP];
-      double acc[NB_GROUP];
-      bool bad[NB_GROUP];
-#pragma unroll
-      for (int u = 0; u < NB_GROUP; u++) {
-        np[u] = 0;
-        acc[u] = 0.;
-        bad[u] = false;
-      }
-      for (int b0 = 0; b0 < T; b0 += 64) {
-        const int q = b0 + lane;
-        const bool valid = q < T;
-        const int j = valid ? cand_at(q) : g0;
-        const double4 pj = nl4[j];
-        double2 tr = make_double2(0., 1.);
-        if (PSE_MODE) tr = pse2[j];
-        const bool cand_ok = valid && pj.w == 0.;
-        const bool cand_bad = valid && pj.w == 2.;  // a non-finite neighbour record
-#pragma unroll
-        for (int u = 0; u < NB_GROUP; u++) {
-          if (u < gn) {
-            const NbParticle &d = s_pd[w][u];
-            const double xij = d.x - pj.x, yij = d.y - pj.y, zij = d.z - pj.z;
-            const double rr2 = xij * xij + yij * yij + zij * zij;
-            const bool inside = !(rr2 >= d.fourh2 || rr2 < 1e-8);  // src/kernels.cuh:41
-            // lapl_pse (src/kernels.cuh:71-92) times (T_j - T_i) m / rho_j
-            // (src/interactions_gpu.cu:124-147); r^2 / h^2 as r^2 times 1 / h^2 and m / rho_j
-            // from the record, each within two ulps of the reference's sqrt-square and divisions
-            if (PSE_MODE == 1 && valid) {
-              const double w2 = d.coef * exp(-rr2 * d.inv_h2);
-              acc[u] += (tr.x - d.Ti) * w2 * tr.y;
-            }
-            if (PSE_MODE == 2 && valid) {
-              const float w2 = (float)d.coef * __expf(-(float)rr2 * (float)d.inv_h2);
-              acc[u] += (double)((float)(tr.x - d.Ti) * w2 * (float)tr.y);
-            }
-            bad[u] = bad[u] || cand_bad;
-            const bool ok = cand_ok && !d.blanked && !d.nonfinite && inside;
-            const unsigned long long m = __ballot(ok);
-            const int pos = np[u] + __popcll(m & ((1ull << lane) - 1ull));
-            if (ok && pos < cap) nbr[(size_t)(g0 + u) * cap + pos] = j;
-            np[u] += __popcll(m);
-          }
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < NB_GROUP; u++) {
-        if (u < gn) {
-          double a = acc[u];
-          if (PSE_MODE) {
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
-          }
-          const int p = g0 + u;
-          const bool poisoned = __ballot(bad[u]) != 0;
-          if (lane == 0) {
-            poison[p] = poisoned ? 1 : 0;
-            if (np[u] > cap) {
-              atomicOr(&cnt->err_capacity, 1);
-              atomicMax(&cnt->max_count, np[u]);
-            }
-            count[p] = np[u] > cap ? cap : np[u];
-            if (PSE_MODE) T_t[p] = ((flags[p] & FL_TOOL) ? c.alpha_tool : c.alpha_wp) * a;
-          }
-          n_cand += (unsigned long long)T;
-          n_pair += (unsigned long long)(np[u] > cap ? cap : np[u]);
-        }
-      }
-      wave_lds_sync();  // the next group overwrites s_pd
-    }
+    cell_lists<PSE_MODE>(s, e, T, lane, s_pd[w], fetch, nl4, pse2, h, flags, c, nbr, cap, count,
+                         T_t, cnt, poison, n_cand, n_pair);
   }
   // candidate / pair totals of this workgroup (work accounting of the roofline figures)
   if (lane == 0) {
