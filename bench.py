@@ -249,7 +249,7 @@ def sweep(args) -> None:
         workload = (f"served: Poisson arrivals 4/s over {args.served:g} s, N log-uniform in [2e4, 2e6] (4:2:1 blocks), "
                     f"70% FP32+Kahan / 30% FP64, 100-1000 steps, parameters as the batched form")
     log(f"{len(reqs)} requests, {sum(r.lattice()[0] * r.lattice()[1] * r.lattice()[2] for r in reqs)} particles in all")
-    res = serve.serve(reqs, hbm_budget=args.hbm_budget_gb * 1e9, log=log)
+    res = serve.serve(reqs, hbm_budget=args.hbm_budget_gb * 1e9, log=log, streams=args.streams)
     log(f"p50 {res['p50_s']:.2f} s, p99 {res['p99_s']:.2f} s, {res['particle_steps_per_s']:.4e} particle-steps/s "
         f"over {res['makespan_s']:.1f} s")
     out = {
@@ -258,7 +258,8 @@ def sweep(args) -> None:
         "steps": int(sum(r.steps for r in reqs)), "warmup": 0, "ms_per_step": None, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "mixed" if args.served > 0 else "f32",
         "data": "synthetic (seeded lattices + jitter, wedge tools)",
-        "config": {"workload": workload, "requests": len(reqs), "hbm_budget_gb": args.hbm_budget_gb},
+        "config": {"workload": workload, "requests": len(reqs), "hbm_budget_gb": args.hbm_budget_gb,
+                   "streams": args.streams},
         "sweep": {"p50_time_to_result_s": res["p50_s"], "p99_time_to_result_s": res["p99_s"],
                   "makespan_s": res["makespan_s"], "particle_steps": res["particle_steps"]},
     }
@@ -287,6 +288,8 @@ def main():
                     help="batched form instead: this many instances at time zero, N log-uniform in "
                          "[5e3, 2e5], FP32+Kahan, 300 steps")
     ap.add_argument("--hbm-budget-gb", type=float, default=250.0)
+    ap.add_argument("--streams", type=int, default=4,
+                    help="served / batched forms: streams the instances are spread over")
     ap.add_argument("--error-report", action="store_true",
                     help="precision form instead: run the workload --steps steps in FP64 and in both FP32 "
                          "modes and report the FP32 L2 and max errors against the FP64 run")

@@ -12,8 +12,9 @@
  *     (the reference instead prints and calls exit(), src/types.cpp:20-49)
  *   - arrays given to *_create are host arrays in the reference's layout (float4_t positions and
  *     velocities = 4 doubles per particle, mat3x3_t = 9 doubles in GLM column-major order)
- *   - all device work is queued on one HIP stream owned by the library (mfg_get_stream);
- *     calls that return host data synchronise that stream
+ *   - all device work is queued on one HIP stream owned by the library (mfg_get_stream; the
+ *     selected stream of a pool, mfg_select_stream); calls that return host data synchronise
+ *     that stream
  *   - particle fields are stored in the current sorted order; they move every step
  *     (as the reference's Green grid swaps its buffers, src/grid_gpu_green.cu:242-263), so a
  *     caller never caches device pointers across steps and reads fields with
@@ -132,6 +133,19 @@ int mfg_init(int device);
 const char *mfg_last_error(void);
 int mfg_synchronize(void);                       /* check_cuda_error, src/types.cpp:20-49 */
 void *mfg_get_stream(void);                      /* hipStream_t of the library */
+/* Streams for independent instances.  No counterpart in the reference, which runs one instance
+ * per process on the default stream: these are for callers that step many independent cutting
+ * instances in one process (the served and batched forms), so that the small grids of
+ * different instances run on the GPU at the same time.  mfg_stream_pool(n) makes n streams
+ * (1 .. MFG_MAX_STREAMS; stream 0 is the library stream, streams are kept until the process
+ * ends); mfg_select_stream(k) makes stream k the one every later call queues on, and the one
+ * mfg_synchronize and the host-data calls wait for.  All objects of one instance must be used
+ * on one stream, and destroyed with that stream selected; instances on different streams share
+ * no device state.  mfg_device_synchronize waits for every stream of the pool. */
+#define MFG_MAX_STREAMS 16
+int mfg_stream_pool(int n);
+int mfg_select_stream(int k);
+int mfg_device_synchronize(void);
 int mfg_set_options(const mfg_options *opt);     /* src/types.h:32-108 */
 /* Clears every constant set by the *_setup_* calls below, and the switches they turn on (thermal,
  * friction, plasticity), to the state after mfg_init.  No counterpart in the reference, whose

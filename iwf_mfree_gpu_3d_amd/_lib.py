@@ -106,6 +106,9 @@ SIGNATURES = {
     "mfg_last_error": (C.c_char_p, []),
     "mfg_synchronize": (_i, []),
     "mfg_get_stream": (_vp, []),
+    "mfg_stream_pool": (_i, [_i]),
+    "mfg_select_stream": (_i, [_i]),
+    "mfg_device_synchronize": (_i, []),
     "mfg_set_options": (_i, [C.POINTER(options)]),
     "mfg_get_options": (_i, [C.POINTER(options)]),
     "mfg_version": (_i, []),

@@ -523,6 +523,21 @@ def synchronize() -> None:
     check(_L().mfg_synchronize())
 
 
+def stream_pool(n: int) -> None:
+    """n streams for independent instances (stream 0 is the library stream)."""
+    check(_L().mfg_stream_pool(n))
+
+
+def select_stream(k: int) -> None:
+    """Stream k of the pool takes every later call (and synchronize waits for it)."""
+    check(_L().mfg_select_stream(k))
+
+
+def device_synchronize() -> None:
+    """Waits for every stream of the pool."""
+    check(_L().mfg_device_synchronize())
+
+
 # ---- tool mesh readers ---------------------------------------------------------------------
 def _read_mesh(call, *args):
     nv, nt = C.c_int(0), C.c_int(0)

@@ -40,7 +40,9 @@ namespace {
 struct Lib {
   bool ready = false;
   int device = -1;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // the stream calls queue on: streams[k] of mfg_select_stream
+  hipStream_t streams[MFG_MAX_STREAMS] = {};
+  int n_streams = 0;
   mfg_options opt{0, MFG_HEAT_BROOKSHAW, 1, 256, MFG_PAIR_LIST, 0};
   // exhaustive contact: closest triangle found by one wavefront per (particle, tool) pair;
   // MFG_LANE_CONTACT=0 in the environment selects the one-thread-per-particle search instead
@@ -603,6 +605,8 @@ int mfg_init(int device) {
   if (device < 0 || device >= n) return fail(MFG_ERR_ARG, "device %d of %d", device, n);
   HIP_TRY(hipSetDevice(device));
   HIP_TRY(hipStreamCreateWithFlags(&g_lib.stream, hipStreamNonBlocking));
+  g_lib.streams[0] = g_lib.stream;
+  g_lib.n_streams = 1;
   const char *lane_contact = std::getenv("MFG_LANE_CONTACT");
   g_lib.opt_contact_search = !(lane_contact && lane_contact[0] == '0');
   g_lib.device = device;
@@ -615,6 +619,29 @@ void *mfg_get_stream(void) { return (void *)g_lib.stream; }
 int mfg_synchronize(void) {
   NEED_LIB();
   HIP_TRY(hipStreamSynchronize(g_lib.stream));
+  return MFG_OK;
+}
+
+int mfg_stream_pool(int n) {
+  NEED_LIB();
+  if (n < 1 || n > MFG_MAX_STREAMS) return fail(MFG_ERR_ARG, "%d streams (1 .. %d)", n, MFG_MAX_STREAMS);
+  // default priority, like the library stream
+  for (int k = g_lib.n_streams; k < n; k++)
+    HIP_TRY(hipStreamCreateWithFlags(&g_lib.streams[k], hipStreamNonBlocking));
+  if (n > g_lib.n_streams) g_lib.n_streams = n;
+  return MFG_OK;
+}
+
+int mfg_select_stream(int k) {
+  NEED_LIB();
+  if (k < 0 || k >= g_lib.n_streams) return fail(MFG_ERR_ARG, "stream %d of %d", k, g_lib.n_streams);
+  g_lib.stream = g_lib.streams[k];
+  return MFG_OK;
+}
+
+int mfg_device_synchronize(void) {
+  NEED_LIB();
+  for (int k = 0; k < g_lib.n_streams; k++) HIP_TRY(hipStreamSynchronize(g_lib.streams[k]));
   return MFG_OK;
 }
 

@@ -11,7 +11,9 @@ its completion time minus its arrival time.
 Instances share no device state.  The library's constants are process-wide (as the reference's
 are), so an instance re-installs its own constants, tools, blanking box and clock before each of
 its steps (after mfg_constants_reset); a result therefore does not depend on the mix of requests
-it ran with.  The batched form is the same loop with every request arriving at time zero.
+it ran with.  Each instance queues its work on one stream of a small pool (the least loaded at
+admission), so the kernels of different instances, whose grids are small, run on the GPU at the
+same time.  The batched form is the same loop with every request arriving at time zero.
 """
 from __future__ import annotations
 
@@ -94,8 +96,10 @@ class Instance:
 
     DX = 5e-4  # lattice spacing [cm]
 
-    def __init__(self, req: Request):
+    def __init__(self, req: Request, stream: int = 0):
         self.req = req
+        self.stream = stream
+        api.select_stream(stream)
         nx, ny, nz = req.lattice()
         cfg = _lib.cut_config()
         _lib.check(_lib.load().mfg_scene_preset(1, 1.0, req.seed, C.byref(cfg)))
@@ -138,6 +142,7 @@ class Instance:
         k = min(steps, self.req.steps - self.k)
         if k <= 0:
             return 0
+        api.select_stream(self.stream)
         self.install()
         for _ in range(k):
             self.stepper.step(self.particles, self.grid)
@@ -150,16 +155,25 @@ class Instance:
     def done(self) -> bool:
         return self.k >= self.req.steps
 
+    def finish(self) -> None:
+        """Waits for the instance's queued work (its stream selected for what follows)."""
+        api.select_stream(self.stream)
+        api.synchronize()
+
     def release(self) -> None:
+        api.select_stream(self.stream)
         self.particles = self.grid = self.tools = self.blanking = self.stepper = None
 
 
 def serve(requests: list, hbm_budget: float = 250e9, steps_per_round: int = 5, on_done=None,
-          clock=time.perf_counter, log=None) -> dict:
+          clock=time.perf_counter, log=None, streams: int = 4) -> dict:
     """Runs the request loop until every request has finished.  `on_done(instance)` is called
-    for each finished instance before its state is released.  Returns per-request records and
-    p50 / p99 time to result and the aggregate particle-steps/s."""
+    for each finished instance before its state is released.  `streams`: size of the stream pool
+    the instances are spread over.  Returns per-request records and p50 / p99 time to result and
+    the aggregate particle-steps/s."""
     _lib.init()
+    api.stream_pool(streams)
+    load = [0] * streams  # active instances per stream
     pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
     active = []
     used = 0.0
@@ -173,7 +187,9 @@ def serve(requests: list, hbm_budget: float = 250e9, steps_per_round: int = 5, o
             if active and used + need > hbm_budget:
                 break  # first come, first served: later requests wait behind this one
             r = pending.pop(0)
-            inst = Instance(r)
+            k = min(range(streams), key=lambda q: load[q])
+            load[k] += 1
+            inst = Instance(r, k)
             used += need
             active.append(inst)
             records[r.rid] = {"rid": r.rid, "arrival": r.arrival, "admitted": clock() - t0, "particles": inst.n,
@@ -188,16 +204,18 @@ def serve(requests: list, hbm_budget: float = 250e9, steps_per_round: int = 5, o
         for inst in list(active):
             steps_total += inst.n * inst.advance(steps_per_round)
             if inst.done:
-                api.synchronize()
+                inst.finish()
                 rec = records[inst.req.rid]
                 rec["finished"] = clock() - t0
                 rec["time_to_result"] = rec["finished"] - rec["arrival"]
                 if on_done:
                     on_done(inst)
                 inst.release()
+                load[inst.stream] -= 1
                 active.remove(inst)
                 used -= device_bytes(inst.req)
-    api.synchronize()
+    api.device_synchronize()
+    api.select_stream(0)
     makespan = clock() - t0
     ttr = np.array([r["time_to_result"] for r in records.values()])
     return {"requests": [records[k] for k in sorted(records)],

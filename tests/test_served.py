@@ -32,11 +32,16 @@ class FakeInstance:
     """Stands in for the device state: a request takes `steps` calls of advance(1)."""
     active = 0
     peak = 0
+    streams = []  # stream of each instance at admission, with the instances then active on it
 
-    def __init__(self, req):
-        self.req, self.k, self.n = req, 0, 1000
+    def __init__(self, req, stream=0):
+        self.req, self.k, self.n, self.stream = req, 0, 1000, stream
         FakeInstance.active += 1
         FakeInstance.peak = max(FakeInstance.peak, FakeInstance.active)
+        FakeInstance.streams.append(stream)
+
+    def finish(self):
+        pass
 
     def advance(self, steps):
         k = min(steps, self.req.steps - self.k)
@@ -54,7 +59,10 @@ class FakeInstance:
 def test_admission_respects_the_budget(monkeypatch):
     monkeypatch.setattr(serve, "Instance", FakeInstance)
     monkeypatch.setattr(_lib, "init", lambda *a, **k: None)
-    monkeypatch.setattr(api, "synchronize", lambda: None)
+    for f in ("synchronize", "device_synchronize"):
+        monkeypatch.setattr(api, f, lambda: None)
+    monkeypatch.setattr(api, "stream_pool", lambda n: None)
+    monkeypatch.setattr(api, "select_stream", lambda k: None)
     reqs = serve.sweep_requests(count=12, batched=True, n_min=1e5, n_max=1e5, steps=(20, 20), seed=2)
     per = serve.device_bytes(reqs[0])
     FakeInstance.active = FakeInstance.peak = 0
@@ -69,6 +77,11 @@ def test_admission_respects_the_budget(monkeypatch):
     FakeInstance.active = FakeInstance.peak = 0
     res = serve.serve(reqs[:2], hbm_budget=0.5 * per)
     assert FakeInstance.peak == 1 and len(res["requests"]) == 2
+    # instances admitted together go to different streams of the pool
+    FakeInstance.active = FakeInstance.peak = 0
+    FakeInstance.streams = []
+    serve.serve(reqs[:6], hbm_budget=10 * per, streams=4)
+    assert FakeInstance.streams == [0, 1, 2, 3, 0, 1]
 
 
 FIELDS = [_lib.F_POS, _lib.F_VEL, _lib.F_S, _lib.F_T, _lib.F_EPS_PL, _lib.F_RHO]
