@@ -19,6 +19,8 @@
 // Precision: FP64 evaluates the reference expressions in the reference order.  FP32 computes
 // the relative position in FP64, the pair arithmetic in FP32 and accumulates every neighbour sum
 // with Kahan compensation; the particle update stays FP64.
+#include <type_traits>
+
 #include "mfg_device.hpp"
 
 namespace mfg {
@@ -79,7 +81,9 @@ __device__ __forceinline__ void wave_lds_sync() {
 struct NbParticle {
   double x, y, z, fourh2;
   double Ti, coef, inv_h2;
-  int blanked, nonfinite;  // nonfinite: its own record holds a non-finite value (no list)
+  float coef_f, inv_h2_f;  // FP32 copies for PSE_MODE 2
+  int nolist;              // blanked, or its own record holds a non-finite value: no list
+  int pad;
 };
 
 // Candidate q of a cell's list as the neighbour pass needs it: particle index, (x, y, z, flag)
@@ -112,8 +116,8 @@ __device__ __forceinline__ void cell_lists(int s, int e, int T, int lane, NbPart
       NbParticle d;
       d.x = pi.x; d.y = pi.y; d.z = pi.z;
       d.fourh2 = 4 * hi * hi;
-      d.blanked = pi.w == 1. ? 1 : 0;
-      d.nonfinite = pi.w == 2. ? 1 : 0;
+      d.nolist = pi.w != 0. ? 1 : 0;
+      d.pad = 0;
       d.Ti = 0.; d.coef = 0.; d.inv_h2 = 0.;
       if (PSE_MODE) {
         d.Ti = pse2[p].x;
@@ -123,27 +127,40 @@ __device__ __forceinline__ void cell_lists(int s, int e, int T, int lane, NbPart
         d.coef = +4. / (h5 * PI_D * sqrt_rn(PI_D));
         d.inv_h2 = 1. / h2;
       }
+      d.coef_f = (float)d.coef;
+      d.inv_h2_f = (float)d.inv_h2;
       pd[lane] = d;
     }
     wave_lds_sync();
+    // per particle of the group: list length so far, PSE partial, whether it gets a list, its
+    // row (all wave-uniform but the PSE partial)
     int np[NB_GROUP];
     double acc[NB_GROUP];
-    bool bad[NB_GROUP];
+    bool listed[NB_GROUP];
+    int *row[NB_GROUP];
 #pragma unroll
     for (int u = 0; u < NB_GROUP; u++) {
       np[u] = 0;
       acc[u] = 0.;
-      bad[u] = false;
+      listed[u] = u < gn && __builtin_amdgcn_readfirstlane(pd[u < gn ? u : 0].nolist) == 0;
+      row[u] = nbr + (size_t)(g0 + u) * cap;
     }
-    for (int b0 = 0; b0 < T; b0 += 64) {
+    // a non-finite record among the candidates poisons every particle of the group (they share
+    // the stencil)
+    unsigned long long bad = 0;
+    // one chunk of 64 candidates against the group; TAIL: the last chunk, lanes past T masked
+    auto chunk = [&](const int b0, auto tail) {
+      constexpr bool TAIL = decltype(tail)::value;
       const int q = b0 + lane;
-      const bool valid = q < T;
+      const bool valid = !TAIL || q < T;
       const NbCand cd = fetch(q, valid, g0);
       const int j = cd.j;
       const double4 pj = cd.pj;
       const double2 tr = cd.tr;
       const bool cand_ok = valid && pj.w == 0.;
-      const bool cand_bad = valid && pj.w == 2.;  // a non-finite neighbour record
+      bad |= __builtin_amdgcn_ballot_w64(valid && pj.w == 2.);
+      float try_f = 0.f;
+      if (PSE_MODE == 2) try_f = (float)tr.y;
 #pragma unroll
       for (int u = 0; u < NB_GROUP; u++) {
         if (u < gn) {
@@ -153,24 +170,31 @@ __device__ __forceinline__ void cell_lists(int s, int e, int T, int lane, NbPart
           const bool inside = !(rr2 >= d.fourh2 || rr2 < 1e-8);  // src/kernels.cuh:41
           // lapl_pse (src/kernels.cuh:71-92) times (T_j - T_i) m / rho_j
           // (src/interactions_gpu.cu:124-147); r^2 / h^2 as r^2 times 1 / h^2 and m / rho_j
-          // from the record, each within two ulps of the reference's sqrt-square and divisions
-          if (PSE_MODE == 1 && valid) {
+          // from the record, each within two ulps of the reference's sqrt-square and
+          // divisions.  Masked lanes add zero (adding a zero leaves the sum as it is).
+          double t = 0.;
+          if (PSE_MODE == 1) {
             const double w2 = d.coef * exp(-rr2 * d.inv_h2);
-            acc[u] += (tr.x - d.Ti) * w2 * tr.y;
+            t = (tr.x - d.Ti) * w2 * tr.y;
           }
-          if (PSE_MODE == 2 && valid) {
-            const float w2 = (float)d.coef * __expf(-(float)rr2 * (float)d.inv_h2);
-            acc[u] += (double)((float)(tr.x - d.Ti) * w2 * (float)tr.y);
+          if (PSE_MODE == 2) {
+            const float w2 = d.coef_f * __expf(-(float)rr2 * d.inv_h2_f);
+            t = (double)((float)(tr.x - d.Ti) * w2 * try_f);
           }
-          bad[u] = bad[u] || cand_bad;
-          const bool ok = cand_ok && !d.blanked && !d.nonfinite && inside;
-          const unsigned long long m = __ballot(ok);
-          const int pos = np[u] + __popcll(m & ((1ull << lane) - 1ull));
-          if (ok && pos < cap) nbr[(size_t)(g0 + u) * cap + pos] = j;
+          if (PSE_MODE) acc[u] += TAIL ? (valid ? t : 0.) : t;
+          // ordered compaction: the kept candidates of the chunk go to the row in lane order
+          const bool ok = cand_ok && listed[u] && inside;
+          const unsigned long long m = __builtin_amdgcn_ballot_w64(ok);
+          const unsigned pos =
+              __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+          if (ok && (int)pos < cap - np[u]) row[u][np[u] + (int)pos] = j;
           np[u] += __popcll(m);
         }
       }
-    }
+    };
+    int b0 = 0;
+    for (; b0 + 64 <= T; b0 += 64) chunk(b0, std::false_type{});
+    if (b0 < T) chunk(b0, std::true_type{});
 #pragma unroll
     for (int u = 0; u < NB_GROUP; u++) {
       if (u < gn) {
@@ -180,9 +204,8 @@ __device__ __forceinline__ void cell_lists(int s, int e, int T, int lane, NbPart
           for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
         }
         const int p = g0 + u;
-        const bool poisoned = __ballot(bad[u]) != 0;
         if (lane == 0) {
-          poison[p] = poisoned ? 1 : 0;
+          poison[p] = bad != 0 ? 1 : 0;
           if (np[u] > cap) {
             atomicOr(&cnt->err_capacity, 1);
             atomicMax(&cnt->max_count, np[u]);
@@ -239,8 +262,18 @@ __device__ __forceinline__ int range_of(const int *pref, int nr, int q) {
   return lo;
 }
 
+// minimum waves per SIMD the kernel is compiled for (register budget): FP64 PSE 6 (80 VGPRs; a
+// 40-byte spill touched once per particle group, not in the full-chunk loop), the others 8;
+// faster than the 5 / 7 waves the unconstrained register use gives
+#ifndef NB_WAVES_FP64_PSE
+#define NB_WAVES_FP64_PSE 6
+#endif
+#ifndef NB_WAVES_OTHER
+#define NB_WAVES_OTHER 8
+#endif
 template <int PSE_MODE>
 __global__ void __launch_bounds__(NB_BLOCK)
+    __attribute__((amdgpu_waves_per_eu(PSE_MODE == 1 ? NB_WAVES_FP64_PSE : NB_WAVES_OTHER)))
     k_cell_neighbours(GridInfo g, const int *__restrict__ off, const double4 *__restrict__ nl4,
                       const double2 *__restrict__ pse2, const double *__restrict__ h,
                       const uint32_t *__restrict__ flags, Consts c, int *__restrict__ nbr, int cap,
