@@ -89,7 +89,9 @@ __global__ void __launch_bounds__(BS)
       bool in = px >= b0 && px <= b3 && py >= b1 && py <= b4 && pz >= b2 && pz <= b5;
       uint32_t nf = fl;
       if (too_fast || !in) nf |= FL_BLANKED;
-      else if (!rates_invalid(st, i)) nf &= ~FL_BLANKED;
+      // inside and slow: finite rates unblank (src/actions_gpu.cu:506-529), non-finite ones keep
+      // the flag as it is, so the rates matter only for a blanked particle
+      else if ((fl & FL_BLANKED) && !rates_invalid(st, i)) nf &= ~FL_BLANKED;
       if (nf != fl) st.flags[i] = nf;
       fl = nf;
     }
