@@ -50,6 +50,88 @@ constexpr int BSP = 256;
 #endif
 
 // ---------------------------------------------------------------------------------------------
+// exp(a) for a <= 0, table-driven (P. T. P. Tang, "Table-driven implementation of the exponential
+// function in IEEE floating-point arithmetic", ACM TOMS 15(2), 1989): a = (64 m + j) ln2/64 + r
+// with n = 64 m + j = rint(a 64/ln2) and |r| <= ln2/128 (two-part ln2/64, fused), then
+// exp(a) = 2^m 2^(j/64) (1 + p(r)), p the degree-5 Taylor polynomial of exp(r) - 1 (truncation
+// r^6/720 < 3.5e-17 at |r| = ln2/128).  2^(j/64) is held as lead + trail, and the result is
+// lead + (lead p + trail): within 0.85 ulp (the general library routine: 1 ulp), in 19 vector
+// instructions and one 16-byte LDS read against that routine's 32, which also handles a > 0 and
+// the overflow range.  a is clamped at -746, where exp already rounds to 0; this keeps n in int
+// range.  A NaN argument (only from a non-finite record, whose stencil the neighbour pass poisons)
+// gives 0, not NaN.  tab: EXP2_J64 in LDS.
+// EXP2_J64[j] = {lead, trail}: lead = 2^(j/64) and trail = 2^(j/64) - lead, and the ln2
+// constants, each correctly rounded (checked against a 60-digit evaluation by
+// tests/test_exp_table.py, which reads them from this file).
+// BEGIN EXP TABLE
+constexpr double EXP_INV_LN2_64 = 0x1.71547652b82fep+6;  // 64 / ln 2
+constexpr double EXP_LN2_64_HI = 0x1.62e42fefa39efp-7;   // ln 2 / 64, leading part
+constexpr double EXP_LN2_64_LO = 0x1.abc9e3b39803fp-62;  // ln 2 / 64 - EXP_LN2_64_HI
+__constant__ double2 EXP2_J64[64] = {
+    {0x1.0000000000000p+0, 0x0p+0}, {0x1.02c9a3e778061p+0, -0x1.19083535b085dp-56},
+    {0x1.059b0d3158574p+0, 0x1.d73e2a475b465p-55}, {0x1.0874518759bc8p+0, 0x1.186be4bb284ffp-57},
+    {0x1.0b5586cf9890fp+0, 0x1.8a62e4adc610bp-54}, {0x1.0e3ec32d3d1a2p+0, 0x1.03a1727c57b53p-59},
+    {0x1.11301d0125b51p+0, -0x1.6c51039449b3ap-54}, {0x1.1429aaea92de0p+0, -0x1.32fbf9af1369ep-54},
+    {0x1.172b83c7d517bp+0, -0x1.19041b9d78a76p-55}, {0x1.1a35beb6fcb75p+0, 0x1.e5b4c7b4968e4p-55},
+    {0x1.1d4873168b9aap+0, 0x1.e016e00a2643cp-54}, {0x1.2063b88628cd6p+0, 0x1.dc775814a8495p-55},
+    {0x1.2387a6e756238p+0, 0x1.9b07eb6c70573p-54}, {0x1.26b4565e27cddp+0, 0x1.2bd339940e9d9p-55},
+    {0x1.29e9df51fdee1p+0, 0x1.612e8afad1255p-55}, {0x1.2d285a6e4030bp+0, 0x1.0024754db41d5p-54},
+    {0x1.306fe0a31b715p+0, 0x1.6f46ad23182e4p-55}, {0x1.33c08b26416ffp+0, 0x1.32721843659a6p-54},
+    {0x1.371a7373aa9cbp+0, -0x1.63aeabf42eae2p-54}, {0x1.3a7db34e59ff7p+0, -0x1.5e436d661f5e3p-56},
+    {0x1.3dea64c123422p+0, 0x1.ada0911f09ebcp-55}, {0x1.4160a21f72e2ap+0, -0x1.ef3691c309278p-58},
+    {0x1.44e086061892dp+0, 0x1.89b7a04ef80d0p-59}, {0x1.486a2b5c13cd0p+0, 0x1.3c1a3b69062f0p-56},
+    {0x1.4bfdad5362a27p+0, 0x1.d4397afec42e2p-56}, {0x1.4f9b2769d2ca7p+0, -0x1.4b309d25957e3p-54},
+    {0x1.5342b569d4f82p+0, -0x1.07abe1db13cadp-55}, {0x1.56f4736b527dap+0, 0x1.9bb2c011d93adp-54},
+    {0x1.5ab07dd485429p+0, 0x1.6324c054647adp-54}, {0x1.5e76f15ad2148p+0, 0x1.ba6f93080e65ep-54},
+    {0x1.6247eb03a5585p+0, -0x1.383c17e40b497p-54}, {0x1.6623882552225p+0, -0x1.bb60987591c34p-54},
+    {0x1.6a09e667f3bcdp+0, -0x1.bdd3413b26456p-54}, {0x1.6dfb23c651a2fp+0, -0x1.bbe3a683c88abp-57},
+    {0x1.71f75e8ec5f74p+0, -0x1.16e4786887a99p-55}, {0x1.75feb564267c9p+0, -0x1.0245957316dd3p-54},
+    {0x1.7a11473eb0187p+0, -0x1.41577ee04992fp-55}, {0x1.7e2f336cf4e62p+0, 0x1.05d02ba15797ep-56},
+    {0x1.82589994cce13p+0, -0x1.d4c1dd41532d8p-54}, {0x1.868d99b4492edp+0, -0x1.fc6f89bd4f6bap-54},
+    {0x1.8ace5422aa0dbp+0, 0x1.6e9f156864b27p-54}, {0x1.8f1ae99157736p+0, 0x1.5cc13a2e3976cp-55},
+    {0x1.93737b0cdc5e5p+0, -0x1.75fc781b57ebcp-57}, {0x1.97d829fde4e50p+0, -0x1.d185b7c1b85d1p-54},
+    {0x1.9c49182a3f090p+0, 0x1.c7c46b071f2bep-56}, {0x1.a0c667b5de565p+0, -0x1.359495d1cd533p-54},
+    {0x1.a5503b23e255dp+0, -0x1.d2f6edb8d41e1p-54}, {0x1.a9e6b5579fdbfp+0, 0x1.0fac90ef7fd31p-54},
+    {0x1.ae89f995ad3adp+0, 0x1.7a1cd345dcc81p-54}, {0x1.b33a2b84f15fbp+0, -0x1.2805e3084d708p-57},
+    {0x1.b7f76f2fb5e47p+0, -0x1.5584f7e54ac3bp-56}, {0x1.bcc1e904bc1d2p+0, 0x1.23dd07a2d9e84p-55},
+    {0x1.c199bdd85529cp+0, 0x1.11065895048ddp-55}, {0x1.c67f12e57d14bp+0, 0x1.2884dff483cadp-54},
+    {0x1.cb720dcef9069p+0, 0x1.503cbd1e949dbp-56}, {0x1.d072d4a07897cp+0, -0x1.cbc3743797a9cp-54},
+    {0x1.d5818dcfba487p+0, 0x1.2ed02d75b3707p-55}, {0x1.da9e603db3285p+0, 0x1.c2300696db532p-54},
+    {0x1.dfc97337b9b5fp+0, -0x1.1a5cd4f184b5cp-54}, {0x1.e502ee78b3ff6p+0, 0x1.39e8980a9cc8fp-55},
+    {0x1.ea4afa2a490dap+0, -0x1.e9c23179c2893p-54}, {0x1.efa1bee615a27p+0, 0x1.dc7f486a4b6b0p-54},
+    {0x1.f50765b6e4540p+0, 0x1.9d3e12dd8a18bp-54}, {0x1.fa7c1819e90d8p+0, 0x1.74853f3a5931ep-55},
+};
+// END EXP TABLE
+
+// In two halves, so that a wavefront starts the table reads of a whole particle group before it
+// needs the first (waiting on each read where it is issued left the LDS latency exposed: slower
+// than the library routine despite fewer instructions).  exp_reduce: n, r and the table read;
+// exp_finish: the polynomial and the scaling.
+struct ExpPart {
+  double r;
+  int n;
+  double2 s;
+};
+__device__ __forceinline__ ExpPart exp_reduce(double a, const double2 *tab) {
+  a = __builtin_fmax(a, -746.);
+  const double n = __builtin_rint(a * EXP_INV_LN2_64);
+  ExpPart e;
+  e.r = __builtin_fma(-n, EXP_LN2_64_LO, __builtin_fma(-n, EXP_LN2_64_HI, a));
+  e.n = (int)n;
+  e.s = tab[e.n & 63];
+  return e;
+}
+__device__ __forceinline__ double exp_finish(const ExpPart &e) {
+  const double r = e.r;
+  double p = __builtin_fma(r, 1. / 120., 1. / 24.);
+  p = __builtin_fma(p, r, 1. / 6.);
+  p = __builtin_fma(p, r, 0.5);
+  p = __builtin_fma(p, r, 1.);
+  p = p * r;
+  return __builtin_ldexp(e.s.x + __builtin_fma(e.s.x, p, e.s.y), e.n >> 6);
+}
+
+// ---------------------------------------------------------------------------------------------
 // One wavefront per cell.  All particles of a cell share the 25 candidate ranges, so the wave
 // lists the candidates once (in LDS, in the reference's visiting order) and its lanes take
 // candidates: the candidate loads are coalesced.  The cell's particles are taken in groups of up
@@ -105,8 +187,8 @@ __device__ __forceinline__ void cell_lists(int s, int e, int T, int lane, NbPart
                                            const uint32_t *__restrict__ flags, const Consts &c,
                                            int *__restrict__ nbr, int cap, int *__restrict__ count,
                                            double *__restrict__ T_t, Counters *cnt,
-                                           int *__restrict__ poison, unsigned long long &n_cand,
-                                           unsigned long long &n_pair) {
+                                           int *__restrict__ poison, const double2 *exp_tab,
+                                           unsigned long long &n_cand, unsigned long long &n_pair) {
   for (int g0 = s; g0 < e; g0 += NB_GROUP) {
     const int gn = min(NB_GROUP, e - g0);
     if (lane < gn) {
@@ -161,6 +243,12 @@ __device__ __forceinline__ void cell_lists(int s, int e, int T, int lane, NbPart
       bad |= __builtin_amdgcn_ballot_w64(valid && pj.w == 2.);
       float try_f = 0.f;
       if (PSE_MODE == 2) try_f = (float)tr.y;
+      // lapl_pse (src/kernels.cuh:71-92) times (T_j - T_i) m / rho_j
+      // (src/interactions_gpu.cu:124-147); r^2 / h^2 as r^2 times 1 / h^2 and m / rho_j from the
+      // record, each within two ulps of the reference's sqrt-square and divisions.  Masked lanes
+      // add zero (adding a zero leaves the sum as it is).  FP64: the exp of every particle of the
+      // group is reduced (table reads started) in the first loop and finished in the second.
+      ExpPart ex[NB_GROUP];
 #pragma unroll
       for (int u = 0; u < NB_GROUP; u++) {
         if (u < gn) {
@@ -168,20 +256,12 @@ __device__ __forceinline__ void cell_lists(int s, int e, int T, int lane, NbPart
           const double xij = d.x - pj.x, yij = d.y - pj.y, zij = d.z - pj.z;
           const double rr2 = xij * xij + yij * yij + zij * zij;
           const bool inside = !(rr2 >= d.fourh2 || rr2 < 1e-8);  // src/kernels.cuh:41
-          // lapl_pse (src/kernels.cuh:71-92) times (T_j - T_i) m / rho_j
-          // (src/interactions_gpu.cu:124-147); r^2 / h^2 as r^2 times 1 / h^2 and m / rho_j
-          // from the record, each within two ulps of the reference's sqrt-square and
-          // divisions.  Masked lanes add zero (adding a zero leaves the sum as it is).
-          double t = 0.;
-          if (PSE_MODE == 1) {
-            const double w2 = d.coef * exp(-rr2 * d.inv_h2);
-            t = (tr.x - d.Ti) * w2 * tr.y;
-          }
+          if (PSE_MODE == 1) ex[u] = exp_reduce(-rr2 * d.inv_h2, exp_tab);
           if (PSE_MODE == 2) {
             const float w2 = d.coef_f * __expf(-(float)rr2 * d.inv_h2_f);
-            t = (double)((float)(tr.x - d.Ti) * w2 * try_f);
+            const double t = (double)((float)(tr.x - d.Ti) * w2 * try_f);
+            acc[u] += TAIL ? (valid ? t : 0.) : t;
           }
-          if (PSE_MODE) acc[u] += TAIL ? (valid ? t : 0.) : t;
           // ordered compaction: the kept candidates of the chunk go to the row in lane order
           const bool ok = cand_ok && listed[u] && inside;
           const unsigned long long m = __builtin_amdgcn_ballot_w64(ok);
@@ -189,6 +269,16 @@ __device__ __forceinline__ void cell_lists(int s, int e, int T, int lane, NbPart
               __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
           if (ok && (int)pos < cap - np[u]) row[u][np[u] + (int)pos] = j;
           np[u] += __popcll(m);
+        }
+      }
+      if (PSE_MODE == 1) {
+#pragma unroll
+        for (int u = 0; u < NB_GROUP; u++) {
+          if (u < gn) {
+            const double w2 = pd[u].coef * exp_finish(ex[u]);
+            const double t = (tr.x - pd[u].Ti) * w2 * tr.y;
+            acc[u] += TAIL ? (valid ? t : 0.) : t;
+          }
         }
       }
     };
@@ -262,9 +352,9 @@ __device__ __forceinline__ int range_of(const int *pref, int nr, int q) {
   return lo;
 }
 
-// minimum waves per SIMD the kernel is compiled for (register budget): FP64 PSE 6 (80 VGPRs; a
-// 40-byte spill touched once per particle group, not in the full-chunk loop), the others 8;
-// faster than the 5 / 7 waves the unconstrained register use gives
+// minimum waves per SIMD the kernel is compiled for (register budget): FP64 PSE 6 (80 VGPRs, no
+// spill; 7 waves spill and are no faster), the others 8; faster than the 5 / 7 waves the
+// unconstrained register use gives
 #ifndef NB_WAVES_FP64_PSE
 #define NB_WAVES_FP64_PSE 6
 #endif
@@ -284,8 +374,13 @@ __global__ void __launch_bounds__(NB_BLOCK)
   __shared__ int s_rst[NB_WAVES][32];
   __shared__ NbParticle s_pd[NB_WAVES][NB_GROUP];
   __shared__ unsigned long long s_stat[NB_WAVES][2];
+  __shared__ double2 s_exp[PSE_MODE == 1 ? 64 : 1];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (PSE_MODE == 1) {
+    if (threadIdx.x < 64) s_exp[threadIdx.x] = EXP2_J64[threadIdx.x];
+    __syncthreads();
+  }
   const long cell = (long)xcd_block() * NB_WAVES + w;
   unsigned long long n_cand = 0, n_pair = 0;
   const int s = cell < g.num_cell ? off[cell] : 0;
@@ -328,7 +423,7 @@ __global__ void __launch_bounds__(NB_BLOCK)
       return cd;
     };
     cell_lists<PSE_MODE>(s, e, T, lane, s_pd[w], fetch, nl4, pse2, h, flags, c, nbr, cap, count,
-                         T_t, cnt, poison, n_cand, n_pair);
+                         T_t, cnt, poison, s_exp, n_cand, n_pair);
   }
   // candidate / pair totals of this workgroup (work accounting of the roofline figures)
   if (lane == 0) {
