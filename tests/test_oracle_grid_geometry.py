@@ -380,6 +380,49 @@ def test_contact_force_law_on_a_plane():
     assert np.all(sim2.field(O.F_FC) == 0.0)
 
 
+@pytest.mark.parametrize("vel", [(10.0, 5.0, 3.0), (2.0, 1.0, 0.5)], ids=["slip", "stick"])
+def test_friction_heat_closed_form(vel):
+    """One particle a depth d under the top face of a resting box tool, moving at v, mu = 0.3.
+    The tool's normals point into the tool (src/tool_3d.cpp:294-360), so the contact normal is
+    n = (0, 0, -1).
+    - Coulomb friction (src/tool_3d_gpu.cu:455-583; the trial force is reset each step):
+      v_r = v - v * n with GLM's component-wise product, = (vx, vy, 2 vz) here; the tangential
+      force is -alpha m v_r / dt (stick) or mu |f_N| along it (slip, |v_r| > mu d / dt).
+    - Friction heat (src/actions_gpu.cu:192-220) raises T by eta dt |f_T| |v_h| / (cp m), where
+      v_h = v - (v . n) (1, 1, 1) subtracts the scalar v . n from every component:
+      (vx + vz, vy + vz, 2 vz) here.
+    Both differ from the tangential velocity (vx, vy, 0); the test fails if either is replaced
+    by it."""
+    phys, corr, joco, trml = _material()
+    trml = O.Trml(trml.cp, trml.tq, trml.k, trml.alpha, trml.T_init, 0.9)   # eta = 0.9
+    v, t = _box_mesh((0.0, 0.0, -1.0), (1.0, 1.0, 0.0))
+    d, dt, alpha, mu = 2e-4, 1e-5, 0.1, 0.3
+    vx, vy, vz = vel
+    sim = O.OracleSim(np.array([[0.31, 0.47, -d]]), vel3=np.array([vel]), rho=np.full(1, 4.43),
+                      T=np.full(1, 300.0), h=np.full(1, 1.5e-3))
+    sim.set_options(cspm=True, heat=O.HEAT_NONE)
+    sim.setup_constants(phys, corr, joco, trml)
+    sim.set_grid_locked(np.array([0.25, 0.4, -0.05]), np.array([0.4, 0.55, 0.05]), 1.5e-3)
+    sim.add_tool(v, t, vel=np.zeros(3), mass=phys.mass, alpha=alpha, algo=0, mu=mu)
+    sim.set_time(dt, dt)
+    assert sim.step(record=True) == 0
+    np.testing.assert_array_equal(sim.field(O.F_N)[0], [0.0, 0.0, -1.0])
+    m = phys.mass
+    depth = d - vz * dt                        # the particle moved at v during the step
+    f_n = m * depth * alpha / dt ** 2
+    np.testing.assert_allclose(sim.field(O.F_FC)[0], [0.0, 0.0, f_n], rtol=1e-9)
+    v_r = np.array([vx, vy, 2.0 * vz])
+    slip = np.linalg.norm(v_r) > mu * depth / dt
+    assert slip == (vx == 10.0)
+    f_t = mu * f_n if slip else alpha * m * np.linalg.norm(v_r) / dt
+    np.testing.assert_allclose(sim.field(O.F_FT)[0], -f_t * v_r / np.linalg.norm(v_r), rtol=1e-6)
+    v_h = np.linalg.norm([vx + vz, vy + vz, 2.0 * vz])
+    dT = 0.9 * dt * f_t * v_h / (trml.cp * m)
+    np.testing.assert_allclose(sim.field(O.F_T)[0] - 300.0, dT, rtol=1e-6)
+    # the tangential speed would give a different rise and force
+    assert abs(np.hypot(vx, vy) / v_h - 1.0) > 0.1
+
+
 def test_boundary_conditions_fixed_particles():
     """Fixed particles end the step with the prescribed velocity, zero contact force and zero
     position / velocity rates; with the thermal boundary condition their temperature is reset to
