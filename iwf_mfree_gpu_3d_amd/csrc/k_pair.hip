@@ -1239,8 +1239,10 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
 // Piece p of lane L's record then sits at 64 quad + 16 ((p - rot) & 3) in block L's
 // instruction, on bank group 4 (quad & 3) + ((p - rot) & 3) = 4 ((L + lam) & 3) + ((p - rot) & 3).
 constexpr int DMA_BLOCK = 1280;  // 64 x 16 bytes, 256-byte aligned blocks
+// S: neighbours per lane per step (S sets of NI blocks).  The CSPM term is light, so its walk
+// waits on the loads of every step: two neighbours per step halve those waits.
 struct CspmPlan {
-  static constexpr int P = 4, NI = 4;
+  static constexpr int P = 4, NI = 4, S = 2;
   unsigned *codes_;  // unused
   int off_;          // this lane's piece offset (the same in every instruction)
   __device__ void prepare(int lane) { off_ = 16 * (((lane & 3) + ((lane >> 4) & 3)) & 3); }
@@ -1251,7 +1253,7 @@ struct CspmPlan {
   }
 };
 struct MainPlan {
-  static constexpr int P = 12, NI = 13;
+  static constexpr int P = 12, NI = 13, S = 1;
   // per instruction: source lane (bits 0-7) and piece offset (bits 8-15), two per word, kept in
   // LDS (word m of lane l at codes + 4 (64 m + l)) to spare the registers
   unsigned *codes_;
@@ -1302,15 +1304,17 @@ constexpr int DMA64_AREA = DMA64_CODES + 64 * 4 * ((MainPlan::NI + 1) / 2);
 #endif
 typedef __attribute__((address_space(3))) void lds_void;
 
-// Walks the lists of a wavefront: step k gives each lane with k < nl the first P pieces of its
-// neighbour k.  Step k + 1's records and the index window of steps k + 2 .. k + 5 are loaded
-// while step k's term runs.
+// Walks the lists of a wavefront: step k gives each lane the first P pieces of its neighbours
+// S k .. S k + S - 1 (those below nl), in list order.  Step k + 1's records are loaded while
+// step k's terms run; the index window of the neighbours 4 m .. 4 m + 3 arrives one step before
+// the step that needs it.
 template <class Plan, class Term>
 __device__ __forceinline__ void dma_list_walk(const Rec64 *__restrict__ rec, const int *row,
                                               int nl, int nmax, unsigned ic, int lane,
                                               char *area, Term &&term) {
-  constexpr int P = Plan::P, NI = Plan::NI;
-  static_assert(NI * DMA_BLOCK <= DMA64_WINDOWS, "area");
+  constexpr int P = Plan::P, NI = Plan::NI, S = Plan::S;
+  static_assert(S * NI * DMA_BLOCK <= DMA64_WINDOWS, "area");
+  static_assert(4 % S == 0, "a step's neighbours lie in one index window");
   typedef int iv4 __attribute__((ext_vector_type(4)));
   char *wslot = area + DMA64_WINDOWS;
   Plan plan;
@@ -1324,40 +1328,51 @@ __device__ __forceinline__ void dma_list_walk(const Rec64 *__restrict__ rec, con
     return q == 0 ? wc.x : q == 1 ? wc.y : q == 2 ? wc.z : wc.w;
   };
   auto issue = [&](int k) {
-    const int j = idx_at(k);
-    int jm[NI];
 #pragma unroll
-    for (int q = 0; q < NI; q++) jm[q] = __shfl(j, plan.src_lane(q, lane), 64);
+    for (int s = 0; s < S; s++) {
+      const int j = idx_at(S * k + s);
+      int jm[NI];
 #pragma unroll
-    for (int q = 0; q < NI; q++) {
-      const char *src = (const char *)rec + (size_t)jm[q] * sizeof(Rec64) + plan.off(q);
-      __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)(area + DMA_BLOCK * q), 16,
-                                       0, 0);
+      for (int q = 0; q < NI; q++) jm[q] = __shfl(j, plan.src_lane(q, lane), 64);
+#pragma unroll
+      for (int q = 0; q < NI; q++) {
+        const char *src = (const char *)rec + (size_t)jm[q] * sizeof(Rec64) + plan.off(q);
+        __builtin_amdgcn_global_load_lds((const void *)src,
+                                         (lds_void *)(area + DMA_BLOCK * (NI * s + q)), 16, 0, 0);
+      }
     }
   };
-  if (nmax > 0) issue(0);
-  for (int k = 0; k < nmax; k++) {
+  const int nsteps = (nmax + S - 1) / S;
+  if (nsteps > 0) issue(0);
+  for (int k = 0; k < nsteps; k++) {
     __builtin_amdgcn_s_waitcnt(0);  // this step's records (and a window) are in LDS
     struct Pieces {
       piece16 p[P];
-    } pc;
+    } pc[S];
 #pragma unroll
-    for (int p = 0; p < P; p++) pc.p[p] = *(const piece16 *)(area + Plan::read_pos(lane, p));
-    const bool next_window = ((k + 1) & 3) == 0 && k + 1 < nmax;
+    for (int s = 0; s < S; s++)
+#pragma unroll
+      for (int p = 0; p < P; p++)
+        pc[s].p[p] = *(const piece16 *)(area + DMA_BLOCK * NI * s + Plan::read_pos(lane, p));
+    const int n1 = S * (k + 1), n2 = S * (k + 2);  // first neighbours of steps k + 1, k + 2
+    const bool next_window = (n1 & 3) == 0 && n1 < nmax;
     iv4 wl = wc;
     if (next_window) wl = *(const iv4 *)(wslot + 16 * lane);
     __builtin_amdgcn_s_waitcnt(0);  // every lane's reads are done before the area is refilled
     wave_sync_lds();
-    if (k + 1 < nmax) {
+    if (n1 < nmax) {
       wc = wl;
       issue(k + 1);
-      // the window of steps k+2 .. k+5, read at step k+1 (a lane past its list reads its first)
-      if (((k + 2) & 3) == 0 && k + 2 < nmax) {
-        const int *src = k + 2 < nl ? row + k + 2 : row;
+      // the window of neighbours n2 .. n2 + 3, read at step k + 1 (a lane past its list reads
+      // its first)
+      if ((n2 & 3) == 0 && n2 < nmax) {
+        const int *src = n2 < nl ? row + n2 : row;
         __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)wslot, 16, 0, 0);
       }
     }
-    if (k < nl) term(pc);
+#pragma unroll
+    for (int s = 0; s < S; s++)
+      if (S * k + s < nl) term(pc[s]);
   }
   __builtin_amdgcn_s_waitcnt(0);
   wave_sync_lds();  // the area is free for the next walk
