@@ -33,8 +33,9 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3
 #              artificial stress 15, stress divergence 63, viscosity 40, XSPH 10, Brookshaw 25)
 #   neighbour pass: 11 per candidate (r^2 and support test) + 10 per candidate for the PSE sum
 #   algorithmic HBM bytes of the pair pass per particle: its own record read once (192 B FP64,
-#   112 B FP32; every neighbour record is a record some particle reads anyway), flags and list
-#   length (8 B), its 25 FP64 neighbour sums written (200 B), and its list (4 B per pair)
+#   112 B FP32; every neighbour record is a record some particle reads anyway), its 32-byte
+#   CSPM entry read once, flags and list length (8 B), its 25 FP64 neighbour sums written
+#   (200 B), and its list (4 B per pair)
 FLOPS_PER_PAIR = 280.0
 FLOPS_PER_CANDIDATE = 11.0
 FLOPS_PER_PSE_CANDIDATE = 10.0
@@ -87,9 +88,10 @@ def gpu_run(args):
 
 
 # algorithmic HBM bytes of one pair pass per particle, besides 4 B per neighbour-list entry:
-#   own pair record (192 B FP64 / 112 B FP32) + count 4 B + flags 4 B + 25 neighbour sums
-#   written (200 B); the particle update that reads the sums back is a kernel of its own
-PAIR_BYTES_PER_PARTICLE = {"fp64": 400.0, "fp32": 320.0}
+#   own pair record (192 B FP64 / 112 B FP32) + CSPM entry 32 B + count 4 B + flags 4 B + 25
+#   neighbour sums written (200 B); the particle update that reads the sums back is a kernel of
+#   its own
+PAIR_BYTES_PER_PARTICLE = {"fp64": 432.0, "fp32": 352.0}
 
 # committed rocprofv3 FETCH_SIZE / WRITE_SIZE summaries: (kernel, preset, scale) -> file
 TRAFFIC_PROFILES = {

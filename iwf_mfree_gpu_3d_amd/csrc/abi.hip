@@ -28,8 +28,8 @@ using namespace mfg;
 namespace mfg {
 void launch_permute_predict_ex(hipStream_t s, const State &src, const State &dst, unsigned n_init,
                                const GridBuffers &gb, const Counters *cnt, const Consts &c,
-                               int predict, void *rec, double4 *nl4, double2 *pse2, double *p_dbg,
-                               double *R_dbg);
+                               int predict, void *rec, double4 *nl4, double2 *pse2, double4 *cs4,
+                               double *p_dbg, double *R_dbg);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -177,6 +177,7 @@ struct mfg_particles {
   void *rec = nullptr;
   double4 *nl4 = nullptr;
   double2 *pse2 = nullptr;
+  double4 *cs4 = nullptr;  // what the CSPM sum reads of each record, 32 bytes
   double *sums = nullptr;  // neighbour sums of the pair pass, PAIR_SUM_COUNT x sum_stride (SoA)
   size_t sum_stride = 0;
   int *trav = nullptr;     // visiting order of the list pair pass (N_init)
@@ -285,7 +286,7 @@ size_t padded(size_t n) { return (n + 63) / 64 * 64; }
 
 void free_particles(mfg_particles *p) {
   void *ptrs[] = {p->arena[0], p->arena[1], p->uarena[0], p->uarena[1], p->rec, p->nl4, p->pse2,
-                  p->sums, p->trav, p->seg_pos, p->poison, p->nbr, p->count, p->key, p->key_sorted, p->order, p->blk, p->cnt, p->partials,
+                  p->cs4, p->sums, p->trav, p->seg_pos, p->poison, p->nbr, p->count, p->key, p->key_sorted, p->order, p->blk, p->cnt, p->partials,
                   p->p_dbg, p->R_dbg, p->flag_d, p->stat_part, p->cs_list, p->cs_n,
                   p->cs_slot, p->cs_res};
   for (void *q : ptrs)
@@ -396,7 +397,7 @@ void permute(mfg_grid *g, mfg_particles *p, const Consts &c, int predict) {
   ProfScope ps(predict ? "permute_predict" : "permute");
   int nxt = 1 - p->cur;
   launch_permute_predict_ex(g_lib.stream, p->buf[p->cur], p->buf[nxt], p->N_init, gb, p->cnt, c,
-                            predict, p->rec, p->nl4, p->pse2, p->p_dbg, p->R_dbg);
+                            predict, p->rec, p->nl4, p->pse2, p->cs4, p->p_dbg, p->R_dbg);
   p->cur = nxt;
 }
 
@@ -778,6 +779,7 @@ int mfg_particle_gpu_create(unsigned N, const double *pos4, const double *vel4, 
   p->contact_blocks = p->nblk;
   if (hipMalloc(&p->rec, rec_bytes * n) != hipSuccess || hipMalloc(&p->nl4, sizeof(double4) * n) != hipSuccess ||
       hipMalloc(&p->pse2, sizeof(double2) * n) != hipSuccess ||
+      hipMalloc(&p->cs4, sizeof(double4) * n) != hipSuccess ||
       hipMalloc(&p->sums, sizeof(double) * PAIR_SUM_COUNT * n) != hipSuccess ||
       hipMalloc(&p->trav, sizeof(int) * n) != hipSuccess ||
       hipMalloc(&p->poison, sizeof(int) * n) != hipSuccess ||
@@ -1229,7 +1231,7 @@ int mfg_grid_neighbours(mfg_grid *g, mfg_particles *p, int *h_counts, int *h_lis
   HIP_TRY(hipMemcpyAsync(p->key, p->key_sorted, sizeof(int) * p->N_init, hipMemcpyDeviceToDevice, g_lib.stream));
   HIP_TRY(hipStreamSynchronize(g_lib.stream));
   launch_permute_predict_ex(g_lib.stream, st, p->buf[1 - p->cur], p->N_init, gb, p->cnt, c, 1,
-                            p->rec, p->nl4, p->pse2, nullptr, nullptr);
+                            p->rec, p->nl4, p->pse2, p->cs4, nullptr, nullptr);
   p->cur = 1 - p->cur;
   HIP_TRY(hipMemcpyAsync(p->order, saved_order, sizeof(int) * p->N_init, hipMemcpyDeviceToDevice, g_lib.stream));
   HIP_TRY(hipMemcpyAsync(p->key, saved_keys, sizeof(int) * p->N_init, hipMemcpyDeviceToDevice, g_lib.stream));
@@ -1599,7 +1601,7 @@ int mfg_leap_frog_step(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int rec
                  : p->precision == MFG_FP32_KAHAN ? "pair_fp32_kahan"
                                                   : "pair_fp32_wide");
     launch_pair(s, p->N_init, p->st(), p->rec, p->nbr, p->nl_stride, p->count, c, p->cnt, gi,
-                g->count, p->key_sorted, tiled, p->sums, p->sum_stride, trav);
+                g->count, p->key_sorted, tiled, p->sums, p->sum_stride, trav, p->cs4);
   }
   {
     ProfScope ps("particle_update");

@@ -215,6 +215,22 @@ __global__ void __launch_bounds__(BS) k_fixup(const int *off, long num_cell, int
   }
 }
 
+// what the CSPM sum reads of a neighbour, 32 bytes: FP64 (x, y, z, vol), vol = -1 for a tool
+// particle (a volume is positive); FP32 the record's first 32 bytes (x, y, z, {tool, vol})
+template <class RecT>
+__device__ __forceinline__ double4 cspm_entry(const double *xh, double vol, bool tool);
+template <>
+__device__ __forceinline__ double4 cspm_entry<Rec64>(const double *xh, double vol, bool tool) {
+  return make_double4(xh[0], xh[1], xh[2], tool ? -1.0 : vol);
+}
+template <>
+__device__ __forceinline__ double4 cspm_entry<Rec32>(const double *xh, double vol, bool tool) {
+  struct {
+    float tool, vol;
+  } tv = {tool ? 1.f : 0.f, (float)vol};
+  return make_double4(xh[0], xh[1], xh[2], __builtin_bit_cast(double, tv));
+}
+
 template <class RecT>
 __device__ __forceinline__ void store_rec(RecT *rec, unsigned i, const double *xh, double h,
                                           const double *vh, double rho, const double *A,
@@ -262,7 +278,7 @@ template <class RecT>
 __global__ void __launch_bounds__(BS)
     k_permute_predict(State src, State dst, unsigned n_init, const int *order, const int *key,
                       int *key_sorted, const Counters *cnt, Consts c, int predict, RecT *rec,
-                      double4 *nl4, double2 *pse2, double *p_dbg, double *R_dbg) {
+                      double4 *nl4, double2 *pse2, double4 *cs4, double *p_dbg, double *R_dbg) {
   const unsigned i = blockIdx.x * BS + threadIdx.x;
   if (i >= n_init) return;
   const unsigned N = cnt->N;
@@ -336,6 +352,7 @@ __global__ void __launch_bounds__(BS)
   double vol = c.i_mass / rhoh;
   double cs = sqrt_site(c.i_K / rhoh, c.single_sqrt);
   store_rec<RecT>(rec, i, xh, h, vh, rhoh, A, R, Th, vol, cs, tool ? 1.0 : 0.0);
+  cs4[i] = cspm_entry<RecT>(xh, vol, tool);
   // candidate flag of the neighbour pass: 1 blanked, 2 a non-finite value among the fields a
   // neighbour contributes (the reference's loops visit every candidate of the stencil, so
   // 0 * NaN carries such a value into the sums of every particle around it)
@@ -437,22 +454,22 @@ void launch_fixup(hipStream_t s, const GridInfo &g, GridBuffers &gb) {
 
 void launch_permute_predict_ex(hipStream_t s, const State &src, const State &dst, unsigned n_init,
                                const GridBuffers &gb, const Counters *cnt, const Consts &c,
-                               int predict, void *rec, double4 *nl4, double2 *pse2, double *p_dbg,
-                               double *R_dbg) {
+                               int predict, void *rec, double4 *nl4, double2 *pse2, double4 *cs4,
+                               double *p_dbg, double *R_dbg) {
   if (c.precision == MFG_FP64)
     hipLaunchKernelGGL(k_permute_predict<Rec64>, dim3(nblocks(n_init, BS)), dim3(BS), 0, s, src,
                        dst, n_init, gb.order, gb.key, gb.key_sorted, cnt, c, predict, (Rec64 *)rec,
-                       nl4, pse2, p_dbg, R_dbg);
+                       nl4, pse2, cs4, p_dbg, R_dbg);
   else
     hipLaunchKernelGGL(k_permute_predict<Rec32>, dim3(nblocks(n_init, BS)), dim3(BS), 0, s, src,
                        dst, n_init, gb.order, gb.key, gb.key_sorted, cnt, c, predict, (Rec32 *)rec,
-                       nl4, pse2, p_dbg, R_dbg);
+                       nl4, pse2, cs4, p_dbg, R_dbg);
 }
 
 void launch_permute_predict(hipStream_t s, const State &src, const State &dst, unsigned n_init,
                             const GridBuffers &gb, const Counters *cnt, const Consts &c,
-                            int predict, void *rec, double4 *nl4, double2 *pse2) {
-  launch_permute_predict_ex(s, src, dst, n_init, gb, cnt, c, predict, rec, nl4, pse2, nullptr,
+                            int predict, void *rec, double4 *nl4, double2 *pse2, double4 *cs4) {
+  launch_permute_predict_ex(s, src, dst, n_init, gb, cnt, c, predict, rec, nl4, pse2, cs4, nullptr,
                             nullptr);
 }
 

@@ -774,8 +774,16 @@ struct Pair64 {
   __device__ __forceinline__ bool cspm_sum() const { return !tool_nz; }
   __device__ __forceinline__ void cspm_term(const Consts &c, const CRec &g) {
     if (g.q.w == 1.) return;
-    const double volj = g.q.y;
-    double xij = ri.x - g.p.x, yij = ri.y - g.p.y, zij = ri.z - g.p.z;
+    cspm_add(c, g.p.x, g.p.y, g.p.z, g.q.y);
+  }
+  // the same term from the compact entry (x, y, z, vol), vol < 0 marking a tool particle
+  __device__ __forceinline__ void cspm_term(const Consts &c, const double4 &g) {
+    if (g.w < 0.) return;
+    cspm_add(c, g.x, g.y, g.z, g.w);
+  }
+  __device__ __forceinline__ void cspm_add(const Consts &c, double xj, double yj, double zj,
+                                           double volj) {
+    double xij = ri.x - xj, yij = ri.y - yj, zij = ri.z - zj;
     double rr2 = xij * xij + yij * yij + zij * zij;
     double w, wx, wy, wz;
     cubic_spline64(xij, yij, zij, rr2, h1, fac, c.single_sqrt, w, wx, wy, wz);
@@ -1156,7 +1164,7 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
     k_pair32_coop(unsigned n_init, State st, const Rec32 *__restrict__ rec,
                   const int *__restrict__ nbr, int stride, const int *__restrict__ count,
                   Consts c, const Counters *cnt, double *__restrict__ sums, size_t sstride,
-                  const int *__restrict__ trav) {
+                  const int *__restrict__ trav, const double4 *__restrict__ cs4) {
   const unsigned t = xcd_block() * BSP + threadIdx.x;
   const unsigned N = cnt->N;
   const bool in_range = t < N && t < n_init;
@@ -1169,7 +1177,7 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
   const int *row = nbr + (size_t)ic * stride;
   if (c.cspm) {
     if (P.cspm_sum())
-      gather_loop(row, nl, [&](int j) { return *(const double4 *)&rec[j].x; },
+      gather_loop(row, nl, [&](int j) { return cs4[j]; },
                   [&](const double4 &g) { P.cspm_term(c, g); });
     P.cspm_end();
   }
@@ -1221,17 +1229,16 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
 // cycle: at 16 pieces per pair (CSPM sum and main sum) the per-lane pass is bound there.  Here
 // the wavefront loads the records of a step together, P pieces per record and 64 / P records
 // per load instruction, straight into LDS (global_load_lds_dwordx4: no VGPRs hold them in
-// flight), and every lane then reads its own record from LDS.  The CSPM sum walks the lists
-// with the records' first four pieces, the main sum with all twelve.  The sums of a particle
-// are those of k_pair_list<Pair64>: same terms, same order.
+// flight), and every lane then reads its own record from LDS.  The CSPM sum comes first: each
+// lane gathers its neighbours' 32-byte compact entries (x, y, z, vol: cs4), half the bytes of
+// the record line it needs and a quarter of the L2 footprint, so they stay in the L2.  The sums
+// of a particle are those of k_pair_list<Pair64>: same terms, same order.
 //
 // LDS placement.  Load instruction q writes lane l's 16 bytes at DMA_BLOCK q + 16 l.  Each
 // group of four lanes (a quad) loads one 64-byte line of one record, so the L1 serves the quad
 // with one access; which line a quad takes and the order of the four pieces inside it are
 // chosen so that piece p of the records of any 16 consecutive lanes lies on 16 distinct 16-byte
 // bank groups (the lanes of one 16-byte LDS read phase).  With rot(L) = (L >> 2) & 3:
-//   CSPM walk (4 pieces, one line, 16 records per instruction): quad g of instruction q loads
-//   record 16 q + g, lane e of the quad its piece (e + rot) & 3.
 //   main walk (12 pieces, three lines, 5 records per instruction, 15 of 16 quads): line lam of
 //   record 5 q + rr goes to quad c + 4 k, c = (5 q + rr + lam) & 3 and k the rank of rr among
 //   the records of instruction q with a line of class c; lane e of the quad loads its piece
@@ -1239,21 +1246,8 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
 // Piece p of lane L's record then sits at 64 quad + 16 ((p - rot) & 3) in block L's
 // instruction, on bank group 4 (quad & 3) + ((p - rot) & 3) = 4 ((L + lam) & 3) + ((p - rot) & 3).
 constexpr int DMA_BLOCK = 1280;  // 64 x 16 bytes, 256-byte aligned blocks
-// S: neighbours per lane per step (S sets of NI blocks).  The CSPM term is light, so its walk
-// waits on the loads of every step: two neighbours per step halve those waits.
-struct CspmPlan {
-  static constexpr int P = 4, NI = 4, S = 2;
-  unsigned *codes_;  // unused
-  int off_;          // this lane's piece offset (the same in every instruction)
-  __device__ void prepare(int lane) { off_ = 16 * (((lane & 3) + ((lane >> 4) & 3)) & 3); }
-  __device__ int src_lane(int q, int lane) const { return 16 * q + (lane >> 2); }
-  __device__ int off(int) const { return off_; }
-  __device__ static int read_pos(int L, int p) {
-    return DMA_BLOCK * (L >> 4) + 64 * (L & 15) + 16 * ((p - ((L >> 2) & 3)) & 3);
-  }
-};
 struct MainPlan {
-  static constexpr int P = 12, NI = 13, S = 1;
+  static constexpr int P = 12, NI = 13;
   // per instruction: source lane (bits 0-7) and piece offset (bits 8-15), two per word, kept in
   // LDS (word m of lane l at codes + 4 (64 m + l)) to spare the registers
   unsigned *codes_;
@@ -1304,17 +1298,15 @@ constexpr int DMA64_AREA = DMA64_CODES + 64 * 4 * ((MainPlan::NI + 1) / 2);
 #endif
 typedef __attribute__((address_space(3))) void lds_void;
 
-// Walks the lists of a wavefront: step k gives each lane the first P pieces of its neighbours
-// S k .. S k + S - 1 (those below nl), in list order.  Step k + 1's records are loaded while
-// step k's terms run; the index window of the neighbours 4 m .. 4 m + 3 arrives one step before
-// the step that needs it.
+// Walks the lists of a wavefront: step k gives each lane with k < nl the first P pieces of its
+// neighbour k.  Step k + 1's records and the index window of steps k + 2 .. k + 5 are loaded
+// while step k's term runs.
 template <class Plan, class Term>
 __device__ __forceinline__ void dma_list_walk(const Rec64 *__restrict__ rec, const int *row,
                                               int nl, int nmax, unsigned ic, int lane,
                                               char *area, Term &&term) {
-  constexpr int P = Plan::P, NI = Plan::NI, S = Plan::S;
-  static_assert(S * NI * DMA_BLOCK <= DMA64_WINDOWS, "area");
-  static_assert(4 % S == 0, "a step's neighbours lie in one index window");
+  constexpr int P = Plan::P, NI = Plan::NI;
+  static_assert(NI * DMA_BLOCK <= DMA64_WINDOWS, "area");
   typedef int iv4 __attribute__((ext_vector_type(4)));
   char *wslot = area + DMA64_WINDOWS;
   Plan plan;
@@ -1328,51 +1320,40 @@ __device__ __forceinline__ void dma_list_walk(const Rec64 *__restrict__ rec, con
     return q == 0 ? wc.x : q == 1 ? wc.y : q == 2 ? wc.z : wc.w;
   };
   auto issue = [&](int k) {
+    const int j = idx_at(k);
+    int jm[NI];
 #pragma unroll
-    for (int s = 0; s < S; s++) {
-      const int j = idx_at(S * k + s);
-      int jm[NI];
+    for (int q = 0; q < NI; q++) jm[q] = __shfl(j, plan.src_lane(q, lane), 64);
 #pragma unroll
-      for (int q = 0; q < NI; q++) jm[q] = __shfl(j, plan.src_lane(q, lane), 64);
-#pragma unroll
-      for (int q = 0; q < NI; q++) {
-        const char *src = (const char *)rec + (size_t)jm[q] * sizeof(Rec64) + plan.off(q);
-        __builtin_amdgcn_global_load_lds((const void *)src,
-                                         (lds_void *)(area + DMA_BLOCK * (NI * s + q)), 16, 0, 0);
-      }
+    for (int q = 0; q < NI; q++) {
+      const char *src = (const char *)rec + (size_t)jm[q] * sizeof(Rec64) + plan.off(q);
+      __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)(area + DMA_BLOCK * q), 16,
+                                       0, 0);
     }
   };
-  const int nsteps = (nmax + S - 1) / S;
-  if (nsteps > 0) issue(0);
-  for (int k = 0; k < nsteps; k++) {
+  if (nmax > 0) issue(0);
+  for (int k = 0; k < nmax; k++) {
     __builtin_amdgcn_s_waitcnt(0);  // this step's records (and a window) are in LDS
     struct Pieces {
       piece16 p[P];
-    } pc[S];
+    } pc;
 #pragma unroll
-    for (int s = 0; s < S; s++)
-#pragma unroll
-      for (int p = 0; p < P; p++)
-        pc[s].p[p] = *(const piece16 *)(area + DMA_BLOCK * NI * s + Plan::read_pos(lane, p));
-    const int n1 = S * (k + 1), n2 = S * (k + 2);  // first neighbours of steps k + 1, k + 2
-    const bool next_window = (n1 & 3) == 0 && n1 < nmax;
+    for (int p = 0; p < P; p++) pc.p[p] = *(const piece16 *)(area + Plan::read_pos(lane, p));
+    const bool next_window = ((k + 1) & 3) == 0 && k + 1 < nmax;
     iv4 wl = wc;
     if (next_window) wl = *(const iv4 *)(wslot + 16 * lane);
     __builtin_amdgcn_s_waitcnt(0);  // every lane's reads are done before the area is refilled
     wave_sync_lds();
-    if (n1 < nmax) {
+    if (k + 1 < nmax) {
       wc = wl;
       issue(k + 1);
-      // the window of neighbours n2 .. n2 + 3, read at step k + 1 (a lane past its list reads
-      // its first)
-      if ((n2 & 3) == 0 && n2 < nmax) {
-        const int *src = n2 < nl ? row + n2 : row;
+      // the window of steps k+2 .. k+5, read at step k+1 (a lane past its list reads its first)
+      if (((k + 2) & 3) == 0 && k + 2 < nmax) {
+        const int *src = k + 2 < nl ? row + k + 2 : row;
         __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)wslot, 16, 0, 0);
       }
     }
-#pragma unroll
-    for (int s = 0; s < S; s++)
-      if (S * k + s < nl) term(pc[s]);
+    if (k < nl) term(pc);
   }
   __builtin_amdgcn_s_waitcnt(0);
   wave_sync_lds();  // the area is free for the next walk
@@ -1382,7 +1363,7 @@ __global__ void __launch_bounds__(BSP, PAIR64D_WAVES)
     k_pair64_dma(unsigned n_init, State st, const Rec64 *__restrict__ rec,
                  const int *__restrict__ nbr, int stride, const int *__restrict__ count,
                  Consts c, const Counters *cnt, double *__restrict__ sums, size_t sstride,
-                 const int *__restrict__ trav) {
+                 const int *__restrict__ trav, const double4 *__restrict__ cs4) {
   typedef Pair64::CRec CRec;
   static_assert(sizeof(Rec64) == 192 && sizeof(CRec) == 64, "record pieces");
   const unsigned t = xcd_block() * BSP + threadIdx.x;
@@ -1399,14 +1380,9 @@ __global__ void __launch_bounds__(BSP, PAIR64D_WAVES)
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   char *area = s_area[w];
   if (c.cspm) {
-    // every lane of the wavefront takes part in the loads; a tool particle (no CSPM sum) walks
-    // an empty list
-    const int nc = P.cspm_sum() ? nl : 0;
-    int ncmax = nc;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) ncmax = max(ncmax, __shfl_xor(ncmax, o, 64));
-    dma_list_walk<CspmPlan>(rec, row, nc, ncmax, ic, lane, area,
-                     [&](const auto &pc) { P.cspm_term(c, __builtin_bit_cast(CRec, pc)); });
+    if (P.cspm_sum())
+      gather_loop(row, nl, [&](int j) { return cs4[j]; },
+                  [&](const double4 &g) { P.cspm_term(c, g); });
     P.cspm_end();
   }
   int nmax = nl;
@@ -1666,7 +1642,7 @@ void launch_neighbours(hipStream_t s, const GridInfo &g, const int *offsets, con
 void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *rec, const int *nbr,
                  int nl_stride, const int *count, const Consts &c, Counters *cnt,
                  const GridInfo &g, const int *offsets, const int *key_sorted, int tiled,
-                 double *sums, size_t sstride, const int *trav) {
+                 double *sums, size_t sstride, const int *trav, const double4 *cs4) {
   if (tiled) {
     const dim3 grid((unsigned)((long)g.nx * g.ny)), block(64);
     if (c.precision == MFG_FP64)
@@ -1686,16 +1662,18 @@ void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *re
   const dim3 grid(nb(n_init, BSP)), block(BSP);
   if (c.precision == MFG_FP64 && PAIR64_COOP)
     hipLaunchKernelGGL(k_pair64_dma, grid, block, 0, s, n_init, st, (const Rec64 *)rec, nbr,
-                       nl_stride, count, c, cnt, sums, sstride, trav);
+                       nl_stride, count, c, cnt, sums, sstride, trav, cs4);
   else if (c.precision == MFG_FP64)
     hipLaunchKernelGGL((k_pair_list<Pair64>), grid, block, 0, s, n_init, st, (const Rec64 *)rec,
                        nbr, nl_stride, count, c, cnt, sums, sstride, trav);
   else if (c.precision == MFG_FP32_KAHAN)
     hipLaunchKernelGGL((k_pair32_coop<AccKahan>), grid, block, 0, s, n_init, st,
-                       (const Rec32 *)rec, nbr, nl_stride, count, c, cnt, sums, sstride, trav);
+                       (const Rec32 *)rec, nbr, nl_stride, count, c, cnt, sums, sstride, trav,
+                       cs4);
   else
     hipLaunchKernelGGL((k_pair32_coop<AccWide>), grid, block, 0, s, n_init, st,
-                       (const Rec32 *)rec, nbr, nl_stride, count, c, cnt, sums, sstride, trav);
+                       (const Rec32 *)rec, nbr, nl_stride, count, c, cnt, sums, sstride, trav,
+                       cs4);
 }
 
 int pair_order_segments(const GridInfo &g, int strip) {

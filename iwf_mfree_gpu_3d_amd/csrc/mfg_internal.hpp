@@ -8,6 +8,8 @@
 //   half-step records (sorted order, rebuilt every step by the permute/predict kernel):
 //     nl4   : double4 (x, y, z, blanked)          -- candidate filter of the neighbour-list pass
 //     pse2  : double2 (T, rho)                    -- PSE heat pass (only with PSE)
+//     cs4   : double4 (x, y, z, vol or -1 for a tool particle) FP64; the first 32 bytes of
+//             the record FP32                     -- CSPM sum of the pair pass
 //     rec   : pair record, 192 B (FP64) / 112 B (FP32) -- everything a neighbour contributes
 //   neighbour list: int32 nbr[i * cap + k] (row-major), count[i]
 //   neighbour sums: 25 doubles per particle (SoA), written by the pair pass and read by the
@@ -56,7 +58,7 @@ constexpr int STATE_DOUBLES = 3 + 3 + 3 + 6 + 2 + 6 + 3 + 3 + 3 + 2 + 6;
 // pair record, FP64: 24 doubles
 struct alignas(16) Rec64 {
   double x, y, z, h;
-  double T, vol, c, tool;  // with x .. h: what the CSPM sum reads, the record's first 64 bytes
+  double T, vol, c, tool;  // with x .. h: what the CSPM sum of the per-lane and tiled forms reads
   double vx, vy, vz, rho;
   double A[6];  // (S - p I) / rho^2 for the diagonal, S / rho^2 off the diagonal
   double R[6];  // artificial stress
@@ -171,7 +173,7 @@ void launch_scatter(hipStream_t s, unsigned n_init, const GridInfo &g, GridBuffe
 void launch_fixup(hipStream_t s, const GridInfo &g, GridBuffers &gb);
 void launch_permute_predict(hipStream_t s, const State &src, const State &dst, unsigned n_init,
                             const GridBuffers &gb, const Counters *cnt, const Consts &c,
-                            int predict, void *rec, double4 *nl4, double2 *pse2);
+                            int predict, void *rec, double4 *nl4, double2 *pse2, double4 *cs4);
 void launch_cells_to_tables(hipStream_t s, const GridInfo &g, const int *offsets, int rothlin,
                             int *start, int *end);
 void launch_bbox_reduce(hipStream_t s, const State &st, const Counters *cnt, unsigned n_init,
@@ -191,7 +193,7 @@ void launch_neighbours(hipStream_t s, const GridInfo &g, const int *offsets, con
 void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *rec, const int *nbr,
                  int nl_stride, const int *count, const Consts &c, Counters *cnt,
                  const GridInfo &g, const int *offsets, const int *key_sorted, int tiled,
-                 double *sums, size_t sum_stride, const int *trav);
+                 double *sums, size_t sum_stride, const int *trav, const double4 *cs4);
 // visiting order of the list pair pass: strips of `strip` grid columns along y, x-row by x-row
 // inside a strip (trav[t] = particle visited at position t < N); seg_pos holds
 // pair_order_segments(g, strip) ints
