@@ -80,9 +80,10 @@ def sweep_requests(count: int | None = None, rate: float = 4.0, duration: float 
 
 
 # device bytes per particle of one instance, from the library's allocations: double-buffered
-# state 656, pair record 192 (FP64) / 112 (FP32), nl4 + pse2 48, neighbour sums 200, 256-entry
-# neighbour list 1024, keys / order / counts / visiting order 20; plus cell tables and tools
-STATE_BYTES = {_lib.MFG_FP64: 2140.0, _lib.MFG_FP32_KAHAN: 2060.0, _lib.MFG_FP32_WIDE: 2060.0}
+# state 656, pair record 192 (FP64) / 112 (FP32), nl4 + pse2 48, CSPM entries 32, neighbour sums
+# 200, 256-entry neighbour list 1024, keys / order / counts / visiting order 20; plus cell tables
+# and tools
+STATE_BYTES = {_lib.MFG_FP64: 2172.0, _lib.MFG_FP32_KAHAN: 2092.0, _lib.MFG_FP32_WIDE: 2092.0}
 
 
 def device_bytes(req: Request) -> float:
