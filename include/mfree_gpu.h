@@ -105,8 +105,9 @@ typedef struct {
   int pair_strip;     /* list pair pass: particles are visited in strips of this many grid
                          columns along y, x-row by x-row inside a strip, which keeps the
                          neighbour records in flight within an XCD's L2; 0 selects 8 (FP64) or
-                         16 (FP32 modes), a negative value the sorted order.  The sums do not
-                         depend on it. */
+                         16 (FP32 modes), or the sorted order when five x-rows of records fit in
+                         2 MB (small instances: the strips would only add two launches), a
+                         negative value the sorted order.  The sums do not depend on it. */
 } mfg_options;
 enum { MFG_PAIR_LIST = 0, MFG_PAIR_TILED = 1 };
 

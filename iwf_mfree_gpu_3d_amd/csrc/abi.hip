@@ -1576,9 +1576,14 @@ int mfg_leap_frog_step(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int rec
     const int tiled = g_lib.opt.pair_pass == MFG_PAIR_TILED ? 1 : 0;
     const GridInfo gi = g->info();
     // strip width: 8 grid columns for 192-byte FP64 records, 16 for 112-byte FP32 records
-    // (measured best on the 1M single-grain cut); negative: the sorted order
+    // (measured best on the 1M single-grain cut); negative: the sorted order.  Automatic: the
+    // sorted order when its window (the records of about five x-rows) fits in half an XCD's
+    // 4 MB L2, where strips gain nothing and cost two launches per step.
+    const double rec_b = p->precision == MFG_FP64 ? 192. : 112.;
+    const bool small = 5. * rec_b * p->N_init / (gi.nx > 0 ? gi.nx : 1) <= 2. * 1024 * 1024;
     const int strip = g_lib.opt.pair_strip > 0    ? g_lib.opt.pair_strip
                       : g_lib.opt.pair_strip < 0  ? 0
+                      : small                     ? 0
                       : p->precision == MFG_FP64 ? 8
                                                  : 16;
     const int *trav = nullptr;
