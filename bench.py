@@ -251,7 +251,15 @@ def sweep(args) -> None:
         workload = (f"served: Poisson arrivals 4/s over {args.served:g} s, N log-uniform in [2e4, 2e6] (4:2:1 blocks), "
                     f"70% FP32+Kahan / 30% FP64, 100-1000 steps, parameters as the batched form")
     log(f"{len(reqs)} requests, {sum(r.lattice()[0] * r.lattice()[1] * r.lattice()[2] for r in reqs)} particles in all")
-    res = serve.serve(reqs, hbm_budget=args.hbm_budget_gb * 1e9, log=log, streams=args.streams)
+    tuning, tune_s = None, None
+    if args.autotune:
+        from iwf_mfree_gpu_3d_amd import autotune
+        t0 = time.perf_counter()
+        tuning = autotune.tune(reqs, log=log)  # before the timed request loop
+        tune_s = time.perf_counter() - t0
+        log(f"autotune: {tune_s:.1f} s")
+    res = serve.serve(reqs, hbm_budget=args.hbm_budget_gb * 1e9, log=log, streams=args.streams,
+                      tuning=tuning)
     log(f"p50 {res['p50_s']:.2f} s, p99 {res['p99_s']:.2f} s, {res['particle_steps_per_s']:.4e} particle-steps/s "
         f"over {res['makespan_s']:.1f} s")
     out = {
@@ -261,10 +269,14 @@ def sweep(args) -> None:
         "scaling": "weak", "vs_baseline": None, "dtype": "mixed" if args.served > 0 else "f32",
         "data": "synthetic (seeded lattices + jitter, wedge tools)",
         "config": {"workload": workload, "requests": len(reqs), "hbm_budget_gb": args.hbm_budget_gb,
-                   "streams": args.streams},
+                   "streams": args.streams, "autotune": bool(args.autotune)},
         "sweep": {"p50_time_to_result_s": res["p50_s"], "p99_time_to_result_s": res["p99_s"],
                   "makespan_s": res["makespan_s"], "particle_steps": res["particle_steps"]},
     }
+    if tuning is not None:
+        out["autotune"] = {"seconds": tune_s, "choices": [{"bucket_log2_particles": b, "precision": p,
+                                                           "pair_pass": c[0], "strip": c[1]}
+                                                          for (b, p), c in sorted(tuning.choice.items())]}
     print(json.dumps(out))
 
 
@@ -290,6 +302,9 @@ def main():
                     help="batched form instead: this many instances at time zero, N log-uniform in "
                          "[5e3, 2e5], FP32+Kahan, 300 steps")
     ap.add_argument("--hbm-budget-gb", type=float, default=250.0)
+    ap.add_argument("--autotune", action="store_true",
+                    help="served / batched forms: pick the pair-pass form and visiting order per size "
+                         "bucket by measurement first (outside the timed loop)")
     ap.add_argument("--streams", type=int, default=4,
                     help="served / batched forms: streams the instances are spread over")
     ap.add_argument("--error-report", action="store_true",

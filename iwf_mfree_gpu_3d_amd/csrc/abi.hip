@@ -37,12 +37,16 @@ void launch_permute_predict_ex(hipStream_t s, const State &src, const State &dst
 // ---------------------------------------------------------------------------------------------
 namespace {
 
+// pool streams created at mfg_init (the library stream included)
+constexpr int MFG_EARLY_STREAMS = 4;
+
 struct Lib {
   bool ready = false;
   int device = -1;
   hipStream_t stream = nullptr;  // the stream calls queue on: streams[k] of mfg_select_stream
   hipStream_t streams[MFG_MAX_STREAMS] = {};
-  int n_streams = 0;
+  int n_streams = 0;  // streams of the pool (mfg_stream_pool)
+  int n_created = 0;  // streams created, >= n_streams
   mfg_options opt{0, MFG_HEAT_BROOKSHAW, 1, 256, MFG_PAIR_LIST, 0};
   // exhaustive contact: closest triangle found by one wavefront per (particle, tool) pair;
   // MFG_LANE_CONTACT=0 in the environment selects the one-thread-per-particle search instead
@@ -608,6 +612,14 @@ int mfg_init(int device) {
   HIP_TRY(hipStreamCreateWithFlags(&g_lib.stream, hipStreamNonBlocking));
   g_lib.streams[0] = g_lib.stream;
   g_lib.n_streams = 1;
+  // The first pool streams are created here, before any blocking copy brings up the runtime's
+  // own stream: streams take the process's hardware queues (4 by default) in creation order, and
+  // a pool created later shared a queue between two of its streams (the batched form ran 23 %
+  // slower when an instance had been set up before the pool).  Default priority, like the
+  // library stream.
+  for (int k = 1; k < MFG_EARLY_STREAMS; k++)
+    HIP_TRY(hipStreamCreateWithFlags(&g_lib.streams[k], hipStreamNonBlocking));
+  g_lib.n_created = MFG_EARLY_STREAMS;
   const char *lane_contact = std::getenv("MFG_LANE_CONTACT");
   g_lib.opt_contact_search = !(lane_contact && lane_contact[0] == '0');
   g_lib.device = device;
@@ -627,8 +639,9 @@ int mfg_stream_pool(int n) {
   NEED_LIB();
   if (n < 1 || n > MFG_MAX_STREAMS) return fail(MFG_ERR_ARG, "%d streams (1 .. %d)", n, MFG_MAX_STREAMS);
   // default priority, like the library stream
-  for (int k = g_lib.n_streams; k < n; k++)
+  for (int k = g_lib.n_created; k < n; k++)
     HIP_TRY(hipStreamCreateWithFlags(&g_lib.streams[k], hipStreamNonBlocking));
+  if (n > g_lib.n_created) g_lib.n_created = n;
   if (n > g_lib.n_streams) g_lib.n_streams = n;
   return MFG_OK;
 }

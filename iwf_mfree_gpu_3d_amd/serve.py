@@ -97,9 +97,10 @@ class Instance:
 
     DX = 5e-4  # lattice spacing [cm]
 
-    def __init__(self, req: Request, stream: int = 0):
+    def __init__(self, req: Request, stream: int = 0, choice: tuple = (api.MFG_PAIR_LIST, 0)):
         self.req = req
         self.stream = stream
+        self.choice = tuple(choice)  # (pair pass, strip) the instance's steps run with
         api.select_stream(stream)
         nx, ny, nz = req.lattice()
         cfg = _lib.cut_config()
@@ -121,7 +122,8 @@ class Instance:
         """The library's constants and the driver globals of this instance."""
         p = self.scene.params
         api.constants_reset()
-        api.set_options(cspm=bool(p.cspm), heat=int(p.heat), single_sqrt=True)
+        api.set_options(cspm=bool(p.cspm), heat=int(p.heat), single_sqrt=True,
+                        pair_pass=int(self.choice[0]), pair_strip=int(self.choice[1]))
         api.actions_setup_corrector_constants(p.corr)
         api.actions_setup_physical_constants(p.phys)
         api.actions_setup_johnson_cook_constants(p.joco)
@@ -167,11 +169,12 @@ class Instance:
 
 
 def serve(requests: list, hbm_budget: float = 250e9, steps_per_round: int = 5, on_done=None,
-          clock=time.perf_counter, log=None, streams: int = 4) -> dict:
+          clock=time.perf_counter, log=None, streams: int = 4, tuning=None) -> dict:
     """Runs the request loop until every request has finished.  `on_done(instance)` is called
     for each finished instance before its state is released.  `streams`: size of the stream pool
-    the instances are spread over.  Returns per-request records and p50 / p99 time to result and
-    the aggregate particle-steps/s."""
+    the instances are spread over.  `tuning`: an autotune.Table whose per-bucket pair-pass
+    choice each instance runs with (None: the library defaults).  Returns per-request records
+    and p50 / p99 time to result and the aggregate particle-steps/s."""
     _lib.init()
     api.stream_pool(streams)
     load = [0] * streams  # active instances per stream
@@ -190,7 +193,9 @@ def serve(requests: list, hbm_budget: float = 250e9, steps_per_round: int = 5, o
             r = pending.pop(0)
             k = min(range(streams), key=lambda q: load[q])
             load[k] += 1
-            inst = Instance(r, k)
+            nx, ny, nz = r.lattice()
+            choice = tuning.get(nx * ny * nz, r.precision) if tuning is not None else (api.MFG_PAIR_LIST, 0)
+            inst = Instance(r, k, choice)
             used += need
             active.append(inst)
             records[r.rid] = {"rid": r.rid, "arrival": r.arrival, "admitted": clock() - t0, "particles": inst.n,

@@ -34,8 +34,8 @@ class FakeInstance:
     peak = 0
     streams = []  # stream of each instance at admission, with the instances then active on it
 
-    def __init__(self, req, stream=0):
-        self.req, self.k, self.n, self.stream = req, 0, 1000, stream
+    def __init__(self, req, stream=0, choice=None):
+        self.req, self.k, self.n, self.stream, self.choice = req, 0, 1000, stream, choice
         FakeInstance.active += 1
         FakeInstance.peak = max(FakeInstance.peak, FakeInstance.active)
         FakeInstance.streams.append(stream)
@@ -109,3 +109,68 @@ def test_result_same_alone_and_in_the_mix(hip_device):
     for f in FIELDS:
         np.testing.assert_array_equal(mix[0][f], alone[0][f], err_msg=f"field {f}")
     assert np.abs(alone[0][_lib.F_VEL]).max() > 0.0
+
+
+def test_autotune_picks_the_fastest_per_bucket(tmp_path):
+    """Per (size bucket, precision) the first request seen is measured with every candidate and
+    the fastest wins (ties: the earlier candidate); instances of untuned sizes take the nearest
+    bucket's choice, untuned precisions the default; the table survives a JSON round trip."""
+    from iwf_mfree_gpu_3d_amd import autotune
+    reqs = serve.sweep_requests(count=40, batched=True, n_min=5e3, n_max=2e5, steps=(10, 10), seed=4,
+                                fp32_share=1.0)
+    seen = []
+
+    def fake(req, cand, steps, warmup):
+        nx, ny, nz = req.lattice()
+        seen.append((autotune.bucket(nx * ny * nz), cand))
+        # small instances prefer the sorted order, large ones strips of 16
+        fast = (api.MFG_PAIR_LIST, -1) if nx * ny * nz < 5e4 else (api.MFG_PAIR_LIST, 16)
+        return 1.0 if cand == fast else 2.0
+
+    table = autotune.tune(reqs, measure=fake)
+    buckets = sorted({autotune.bucket(np.prod(r.lattice())) for r in reqs})
+    # FP32: the visiting orders only (the tiled form is not bit-identical there)
+    assert len(seen) == len(buckets) * 4 and all(c[0] == api.MFG_PAIR_LIST for _, c in seen)
+    assert len(autotune.candidates_for(_lib.MFG_FP64)) == len(autotune.CANDIDATES)
+    for b in buckets:
+        want = (api.MFG_PAIR_LIST, -1) if 2 ** (b + 1) <= 5e4 else None
+        got = table.choice[(b, _lib.MFG_FP32_KAHAN)]
+        if want:
+            assert got == want
+        assert got in autotune.CANDIDATES
+    assert table.get(10 ** 9, _lib.MFG_FP32_KAHAN) == table.choice[(max(buckets), _lib.MFG_FP32_KAHAN)]
+    assert table.get(10, _lib.MFG_FP32_KAHAN) == table.choice[(min(buckets), _lib.MFG_FP32_KAHAN)]
+    assert table.get(10 ** 5, _lib.MFG_FP64) == autotune.DEFAULT
+    path = str(tmp_path / "tune.json")
+    table.save(path)
+    back = autotune.Table.load(path)
+    assert back.choice == table.choice and back.times == table.times
+
+
+@pytest.mark.gpu
+def test_autotuned_batch_same_results(hip_device):
+    """A tuned table (measured on the GPU) changes only the pair-pass form (FP64) and the visiting
+    order, so a request's fields are bit-identical with and without it."""
+    from iwf_mfree_gpu_3d_amd import autotune
+    reqs = serve.sweep_requests(count=2, batched=True, n_min=4e3, n_max=1.2e4, steps=(8, 8), seed=12,
+                                fp32_share=1.0)
+    reqs[1].precision = _lib.MFG_FP64
+    table = autotune.tune(reqs, steps=2, warmup=1)
+    for (b, p), c in table.choice.items():
+        assert c in autotune.candidates_for(p)
+    # force choices other than the default so the comparison exercises other kernels / orders
+    for (b, p) in table.choice:
+        table.choice[(b, p)] = (api.MFG_PAIR_TILED, 0) if p == _lib.MFG_FP64 else (api.MFG_PAIR_LIST, 16)
+
+    def grab(store):
+        def on_done(inst):
+            order = np.argsort(inst.particles.download(_lib.F_ID).astype(np.int64))
+            store[inst.req.rid] = {f: inst.particles.download(f)[order] for f in FIELDS}
+        return on_done
+
+    plain, tuned = {}, {}
+    serve.serve(reqs, on_done=grab(plain), steps_per_round=4)
+    serve.serve(reqs, on_done=grab(tuned), steps_per_round=4, tuning=table)
+    for rid in plain:
+        for f in FIELDS:
+            np.testing.assert_array_equal(tuned[rid][f], plain[rid][f], err_msg=f"request {rid} field {f}")
