@@ -1481,9 +1481,10 @@ int mfg_tool_update(mfg_tool *t, double tc, double dt) {
   ProfScope ps("tool_update");
   if (t->algo == MFG_CONTACT_HASHED) launch_tool_update(g_lib.stream, t->boxed, t->p_init_boxed, t->n_boxed, tc, t->vel);
   else {
-    // the exhaustive search runs over the boxed copies too (grid-accelerated, same result)
-    launch_tool_update(g_lib.stream, t->tris, t->p_init_all, t->n_tris, tc, t->vel);
-    launch_tool_update(g_lib.stream, t->boxed, t->p_init_boxed, t->n_boxed, tc, t->vel);
+    // the exhaustive search runs over the boxed copies too (grid-accelerated, same result);
+    // both arrays move in one launch
+    launch_tool_update(g_lib.stream, t->tris, t->p_init_all, t->n_tris, tc, t->vel, t->boxed,
+                       t->p_init_boxed, t->n_boxed);
   }
   t->mesh_tc = tc;
   for (int k = 0; k < 3; k++) {

@@ -479,10 +479,17 @@ __global__ void k_flip(Tri *tris, int n) {
   }
 }
 
-__global__ void k_tool_update(Tri *tris, const double *p_init, int n, double t, double v0,
-                              double v1, double v2) {
+// two triangle arrays in one launch: threads [0, n) move the first, [n, n + n2) the second
+__global__ void k_tool_update(Tri *tris, const double *p_init, int n, Tri *tris2,
+                              const double *p_init2, int n2, double t, double v0, double v1,
+                              double v2) {
   int k = blockIdx.x * BST + threadIdx.x;
-  if (k >= n) return;
+  if (k >= n) {
+    k -= n;
+    if (k >= n2) return;
+    tris = tris2;
+    p_init = p_init2;
+  }
   const double v[3] = {v0, v1, v2};
 #pragma unroll
   for (int q = 0; q < 3; q++)
@@ -774,10 +781,11 @@ void launch_flip_normals(hipStream_t s, Tri *tris, int n) {
 }
 
 void launch_tool_update(hipStream_t s, Tri *tris, const double *p_init, int n, double t,
-                        const double vel[3]) {
-  if (n > 0)
-    hipLaunchKernelGGL(k_tool_update, dim3(nbk(n)), dim3(BST), 0, s, tris, p_init, n, t, vel[0],
-                       vel[1], vel[2]);
+                        const double vel[3], Tri *tris2, const double *p_init2, int n2) {
+  if (n2 < 0 || !tris2) n2 = 0;
+  if (n + n2 > 0)
+    hipLaunchKernelGGL(k_tool_update, dim3(nbk(n + n2)), dim3(BST), 0, s, tris, p_init, n, tris2,
+                       p_init2, n2, t, vel[0], vel[1], vel[2]);
 }
 
 void launch_contact_search(hipStream_t s, const State &st, unsigned n_init, const ToolSet &ts,

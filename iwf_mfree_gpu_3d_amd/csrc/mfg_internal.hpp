@@ -209,8 +209,10 @@ void launch_particle_update(hipStream_t s, unsigned n_init, const State &st, con
                             size_t sum_stride, const int *poison, const Consts &c,
                             const Counters *cnt);
 
+// moves n triangles (and, in the same launch, n2 more of a second array) to time t
 void launch_tool_update(hipStream_t s, Tri *tris, const double *p_init, int n, double t,
-                        const double vel[3]);
+                        const double vel[3], Tri *tris2 = nullptr, const double *p_init2 = nullptr,
+                        int n2 = 0);
 // exhaustive contact, lane-parallel: list the (particle, tool) pairs inside a tool box, then one
 // wavefront per pair finds the closest triangle; k_contact_bc reads the results
 struct ContactSearch {
