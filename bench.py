@@ -7,9 +7,12 @@ contact, CSPM, Gray artificial stress, Johnson-Cook plasticity), FP64 method.  S
 are built by the library's set-up code (seeded); everything the timed steps touch is resident in
 HBM before timing starts.
 
-One JSON line on stdout: metric, value (particle-steps/s), per-kernel timings from HIP events on
-the library stream over the timed region, the roofline of the dominant kernel, and the CPU
-oracle timed on the host cores on a bounded sample of the same workload.
+One JSON line on stdout: metric, value (particle-steps/s of the 1M FP64 run), per-kernel
+timings from HIP events on the library stream over the timed region, the rooflines of the
+dominant kernel (the pair pass) and of the neighbour pass, the whole step's VALU and HBM
+fractions, the metric's FP32 + Kahan vs FP64 pairs at 1M (same workload) and 8M particles (three
+grains), the p50 / p99 time to result of a short served stream, and the CPU oracle timed on the
+host cores on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -53,26 +56,34 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def gpu_run(args):
+def gpu_run(args, preset=None, scale=None, precision=None, steps=None, warmup=None, scene=None):
+    """Times `steps` steps of one workload after `warmup` untimed ones (HIP events per kernel on the
+    library stream over the timed region)."""
     from iwf_mfree_gpu_3d_amd import _lib, api, scenes
     _lib.init(0)
+    preset = args.preset if preset is None else preset
+    scale = args.scale if scale is None else scale
+    precision = args.precision if precision is None else precision
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     t0 = time.time()
-    scene = scenes.Scene(args.preset, args.scale, args.seed)
-    log(f"scene preset {args.preset} scale {args.scale}: {scene.n} particles "
+    if scene is None:
+        scene = scenes.Scene(preset, scale, args.seed)
+    log(f"scene preset {preset} scale {scale}: {scene.n} particles "
         f"({scene.params.n_workpiece} workpiece), {sum(t[1].shape[0] for t in scene.tools)} tool triangles, "
         f"built in {time.time() - t0:.1f}s")
-    prec = {"fp64": _lib.MFG_FP64, "fp32-kahan": _lib.MFG_FP32_KAHAN, "fp32-wide": _lib.MFG_FP32_WIDE}[args.precision]
+    prec = {"fp64": _lib.MFG_FP64, "fp32-kahan": _lib.MFG_FP32_KAHAN, "fp32-wide": _lib.MFG_FP32_WIDE}[precision]
     parts, grid = scenes.setup_cut(scene, precision=prec, single_sqrt=True)
     lf = api.leap_frog(scene.n, grid.num_cell())
     api.global_time_current = api.global_time_dt
-    scenes.run(lf, parts, grid, args.warmup)
+    scenes.run(lf, parts, grid, warmup)
     api.synchronize()
     n_before = parts.N
     api.profile_reset()
     api.profile_enable(True)
     api.synchronize()
     t0 = time.perf_counter()
-    scenes.run(lf, parts, grid, args.steps)
+    scenes.run(lf, parts, grid, steps)
     api.synchronize()
     t1 = time.perf_counter()
     api.profile_enable(False)
@@ -81,10 +92,12 @@ def gpu_run(args):
     cand, pairs = parts.pair_stats()
     secs = t1 - t0
     n_active = 0.5 * (n_before + n_after)
-    value = n_active * args.steps / secs
+    value = n_active * steps / secs
     kernels = {k: {"launches": v[0], "avg_ms": v[1] / max(v[0], 1), "total_ms": v[1]} for k, v in prof.items()}
-    return dict(scene=scene, value=value, secs=secs, n_active=n_active, kernels=kernels, cand=cand, pairs=pairs,
-                parts=parts, n_particles=scene.n)
+    log(f"  preset {preset} {precision}: {value:.4e} particle-steps/s ({secs / steps * 1e3:.3f} ms/step)")
+    del parts, grid, lf
+    return dict(scene=scene, value=value, secs=secs, steps=steps, n_active=n_active, kernels=kernels, cand=cand,
+                pairs=pairs, n_particles=scene.n, precision=precision, preset=preset, scale=scale)
 
 
 # algorithmic HBM bytes of one pair pass per particle, besides 4 B per neighbour-list entry:
@@ -113,60 +126,105 @@ def measured_traffic(kernel: str, args):
     return int(m.group(1)) if m else None
 
 
-def roofline(res, args, pse: bool):
-    """Roofline of the dominant kernel.  Its bound is arithmetic: the pair pass runs on the vector
-    ALUs (no step of the method is a dense contraction, so no matrix instructions), and on MI355X
-    the FP64 vector and matrix peaks are the same 78.6 TFLOP/s.  The HBM view (algorithmic bytes
-    over the launch time, and the measured traffic) is reported beside it."""
+# algorithmic bytes per particle-step of the whole step (two-phase step, every field once per
+# phase, plus the re-sort every step): FP64 1,062 + 694 B; FP32 pair arithmetic with FP64
+# positions 594 + 398 B
+STEP_BYTES = {"fp64": 1062.0 + 694.0, "fp32": 594.0 + 398.0}
+# FP operations per particle-step outside the pair and neighbour passes: predict, EOS, the 3x3
+# eigen solve and artificial stress, Jaumann rate, the update and the Johnson-Cook return
+FLOPS_PER_PARTICLE_UPDATE = 400.0
+
+
+def kernel_roofline(res, kernel: str, args, pse: bool):
+    """Roofline of one kernel of the step: the pair and neighbour passes are bound by the vector
+    ALU (no step of the method is a dense contraction: no matrix instructions), with the HBM
+    view (algorithmic bytes over the launch time, and the measured traffic) beside it."""
     kernels = res["kernels"]
-    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
-    avg_s = kernels[dom]["avg_ms"] * 1e-3
-    fp64 = args.precision == "fp64"
+    if kernel not in kernels:
+        return None
+    avg_s = kernels[kernel]["avg_ms"] * 1e-3
+    fp64 = res["precision"] == "fp64"
     peak = FP64_VECTOR_PEAK_TFLOPS if fp64 else FP32_VECTOR_PEAK_TFLOPS
     n = res["n_active"]
-    if dom.startswith("pair_"):
+    if kernel.startswith("pair_"):
         flops = FLOPS_PER_PAIR * res["pairs"]
         basis = f"{FLOPS_PER_PAIR:.0f} flop x {res['pairs']} interacting pairs per launch"
         alg_bytes = PAIR_BYTES_PER_PARTICLE["fp64" if fp64 else "fp32"] * n + 4.0 * res["pairs"]
-    elif dom == "neighbours":
+    elif kernel == "neighbours":
         per = FLOPS_PER_CANDIDATE + (FLOPS_PER_PSE_CANDIDATE if pse else 0.0)
         flops = per * res["cand"]
         basis = f"{per:.0f} flop x {res['cand']} candidate pairs per launch"
         alg_bytes = 76.0 * n + 4.0 * res["pairs"]
     else:
-        return dom, None
+        return None
     achieved = flops / avg_s / 1e12
     hbm = alg_bytes / avg_s / 1e9
-    return dom, {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                 "frac": achieved / peak, "traffic": measured_traffic(dom, args), "kernel": dom,
-                 "basis": basis, "avg_launch_ms": kernels[dom]["avg_ms"],
-                 "compute_units": ("FP64 vector ALU; no matrix instructions (the FP64 vector and matrix peaks "
-                                   "of MI355X are both 78.6 TFLOP/s)" if fp64 else
-                                   "FP32 vector ALU, packed FP32 not generated; peak quoted is the published "
-                                   "157.3 TFLOP/s FP32 figure"),
-                 "algorithmic_bytes": alg_bytes, "hbm_achieved_GBps": hbm, "hbm_peak_GBps": HBM_PEAK_GBS,
-                 "hbm_frac": hbm / HBM_PEAK_GBS}
+    return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": measured_traffic(kernel, args), "kernel": kernel,
+            "basis": basis, "avg_launch_ms": kernels[kernel]["avg_ms"],
+            "compute_units": ("FP64 vector ALU (78.6 TFLOP/s published; the FP64 matrix peak is the same "
+                              "figure and no matrix instructions are used)" if fp64 else
+                              "FP32 vector ALU, packed FP32 not generated; peak quoted is the published "
+                              "157.3 TFLOP/s FP32 figure"),
+            "algorithmic_bytes": alg_bytes, "hbm_achieved_GBps": hbm, "hbm_peak_GBps": HBM_PEAK_GBS,
+            "hbm_frac": hbm / HBM_PEAK_GBS}
+
+
+def whole_step(res, pse: bool):
+    """VALU and HBM fractions of the whole step: algorithmic flops (pair terms, candidate tests and
+    PSE terms, per-particle update) and algorithmic bytes per particle-step over the measured
+    time per step."""
+    fp64 = res["precision"] == "fp64"
+    n = res["n_active"]
+    step_s = res["secs"] / res["steps"]
+    flops = (FLOPS_PER_PAIR * res["pairs"] + (FLOPS_PER_CANDIDATE + (FLOPS_PER_PSE_CANDIDATE if pse else 0.0))
+             * res["cand"] + FLOPS_PER_PARTICLE_UPDATE * n)
+    bytes_ = STEP_BYTES["fp64" if fp64 else "fp32"] * n
+    peak = FP64_VECTOR_PEAK_TFLOPS if fp64 else FP32_VECTOR_PEAK_TFLOPS
+    return {"ms_per_step": step_s * 1e3, "valu_TFLOPs": flops / step_s / 1e12, "valu_frac": flops / step_s / 1e12 / peak,
+            "hbm_GBps": bytes_ / step_s / 1e9, "hbm_frac": bytes_ / step_s / 1e9 / HBM_PEAK_GBS,
+            "flops_per_particle_step": flops / n, "bytes_per_particle_step": STEP_BYTES["fp64" if fp64 else "fp32"]}
 
 
 def cpu_baseline(args):
-    """The FP64 oracle on the host cores: the same workload, a bounded number of steps."""
+    """The FP64 oracle (a CPU restatement of the reference's step) built with g++ -O3
+    -march=native -fopenmp for this host, timed on its cores on a bounded sample of the same
+    workload: all cores (OMP_NUM_THREADS, else the CPU count, OMP_PROC_BIND=close) for
+    --cpu-steps steps, and one thread for one step.  Each run is a process of its own, so the
+    OpenMP runtime starts with its thread settings."""
+    import subprocess
     import oracle
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import helpers
-    from iwf_mfree_gpu_3d_amd import scenes
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
-    scene = scenes.Scene(args.preset, args.scale, args.seed)
-    sim = helpers.oracle_from_scene(scene, single_sqrt=True, threads=threads)
-    dt = scene.params.dt
-    sim.set_time(dt, dt)
-    sim.run(1)  # warm-up step (first sort from the unsorted lattice)
-    t0 = time.perf_counter()
-    sim.run(args.cpu_steps)
-    secs = time.perf_counter() - t0
-    n = sim.counts()[0]
-    return {"value": n * args.cpu_steps / secs, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"same workload ({scene.n} particles), {args.cpu_steps} steps after 1 warm-up step, "
-                      f"{secs:.1f} s"}
+    lib_path = oracle.build_native()
+    model = "unknown"
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        m = re.search(r"Model name:\s*(.+)", out)
+        model = m.group(1).strip() if m else model
+    except (OSError, subprocess.SubprocessError):
+        pass
+    nproc = os.cpu_count() or 1
+    all_threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or nproc
+
+    def run(threads, steps):
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", MFG_ORACLE_LIB=lib_path)
+        r = subprocess.run([sys.executable, "-m", "oracle.cpu_bench", "--preset", str(args.preset), "--scale",
+                            str(args.scale), "--seed", str(args.seed), "--steps", str(steps), "--threads",
+                            str(threads)], capture_output=True, text=True, env=env, cwd=ROOT, timeout=900)
+        if r.returncode != 0:
+            raise RuntimeError(f"CPU baseline run failed: {r.stderr[-2000:]}")
+        return json.loads(r.stdout.strip().splitlines()[-1])
+
+    many = run(all_threads, args.cpu_steps)
+    one = run(1, 1) if args.cpu_single_steps > 0 else None
+    res = {"value": many["value"], "unit": "particle-steps/s", "cores": all_threads, "kind": "port",
+           "sample": (f"same workload ({many['particles']} particles): {args.cpu_steps} steps after 1 warm-up "
+                      f"step on {all_threads} threads ({many['seconds']:.1f} s)"
+                      + (f"; 1 step on 1 thread ({one['seconds']:.1f} s)" if one else "")),
+           "build": "g++ -O3 -march=native -fopenmp -ffp-contract=off", "omp_proc_bind": "close",
+           "nproc": nproc, "cpu_model": model}
+    if one:
+        res["value_1_thread"] = one["value"]
+    return res
 
 
 def copy_probe(mib: int) -> None:
@@ -289,9 +347,19 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--precision", choices=["fp64", "fp32-kahan", "fp32-wide"], default="fp64")
-    ap.add_argument("--cpu-steps", type=int, default=10,
-                    help="steps of the CPU oracle timed for cpu_baseline (about 1 s each at 1M particles "
-                         "on 16 threads)")
+    ap.add_argument("--cpu-steps", type=int, default=8,
+                    help="steps of the CPU oracle timed on all cores for cpu_baseline (about 1 s each at "
+                         "1M particles on 16 threads)")
+    ap.add_argument("--cpu-single-steps", type=int, default=1,
+                    help="steps of the CPU oracle timed on one thread (0: skip)")
+    ap.add_argument("--steps-8m", type=int, default=10,
+                    help="timed steps of the 8M-particle three-grain runs (FP64 and FP32 + Kahan)")
+    ap.add_argument("--warmup-8m", type=int, default=3)
+    ap.add_argument("--sweep-seconds", type=float, default=4.0,
+                    help="seconds of Poisson arrivals of the served stream whose p50 / p99 time to result "
+                         "the default line reports (0: skip)")
+    ap.add_argument("--quick", action="store_true",
+                    help="only the headline workload (no FP32 / 8M runs, no served stream)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain copy of this many MiB (profiler counter calibration)")
@@ -327,10 +395,45 @@ def main():
     res = gpu_run(args)
     scene = res["scene"]
     pse = scene.params.heat == 2
-    dom, roof = roofline(res, args, pse)
+    dom = max(res["kernels"], key=lambda k: res["kernels"][k]["total_ms"])
+    roof = kernel_roofline(res, dom, args, pse)
+    nb_roof = kernel_roofline(res, "neighbours", args, pse)
+    step = whole_step(res, pse)
     for k, v in sorted(res["kernels"].items(), key=lambda kv: -kv[1]["total_ms"]):
         log(f"  {k:20s} {v['launches']:6d} launches  avg {v['avg_ms']:.4f} ms  total {v['total_ms']:.1f} ms")
     log(f"GPU: {res['value']:.4e} particle-steps/s over {args.steps} steps ({res['secs']:.3f} s)")
+    precision_pairs, sweep_rec = None, None
+    if not args.quick and args.preset == 2 and args.precision == "fp64":
+        from iwf_mfree_gpu_3d_amd import scenes
+        # the metric's FP32 vs FP64 pairs: the same 1M workload in FP32 + Kahan, and the 8M
+        # three-grain workload in both modes
+        r32 = gpu_run(args, precision="fp32-kahan", scene=scene)
+        scene8 = scenes.Scene(3, 1.0, args.seed)
+        r8 = {p: gpu_run(args, preset=3, scale=1.0, precision=p, steps=args.steps_8m, warmup=args.warmup_8m,
+                         scene=scene8) for p in ("fp64", "fp32-kahan")}
+        precision_pairs = {
+            "1M": {"particles": res["n_particles"], "fp64": res["value"], "fp32_kahan": r32["value"],
+                   "fp32_over_fp64": r32["value"] / res["value"], "steps": args.steps,
+                   "fp32_pair_kernel_ms": r32["kernels"].get("pair_fp32_kahan", {}).get("avg_ms")},
+            "8M": {"particles": scene8.n, "fp64": r8["fp64"]["value"], "fp32_kahan": r8["fp32-kahan"]["value"],
+                   "fp32_over_fp64": r8["fp32-kahan"]["value"] / r8["fp64"]["value"], "steps": args.steps_8m,
+                   "workload": WORKLOADS[(3, 1.0)],
+                   "whole_step_fp64": whole_step(r8["fp64"], True),
+                   "whole_step_fp32_kahan": whole_step(r8["fp32-kahan"], True)},
+        }
+        del scene8
+        if args.sweep_seconds > 0:
+            from iwf_mfree_gpu_3d_amd import serve
+            reqs = serve.sweep_requests(rate=4.0, duration=args.sweep_seconds, seed=args.seed)
+            t0 = time.perf_counter()
+            sres = serve.serve(reqs, hbm_budget=args.hbm_budget_gb * 1e9, streams=args.streams)
+            log(f"served stream: {len(reqs)} requests in {time.perf_counter() - t0:.1f} s, p50 {sres['p50_s']:.2f} s, "
+                f"p99 {sres['p99_s']:.2f} s, {sres['particle_steps_per_s']:.4e} particle-steps/s")
+            sweep_rec = {"workload": (f"served: Poisson arrivals 4/s over {args.sweep_seconds:g} s, N log-uniform in "
+                                      f"[2e4, 2e6] (4:2:1 blocks), 70% FP32+Kahan / 30% FP64, 100-1000 steps"),
+                         "requests": len(reqs), "p50_time_to_result_s": sres["p50_s"],
+                         "p99_time_to_result_s": sres["p99_s"], "particle_steps_per_s": sres["particle_steps_per_s"],
+                         "makespan_s": sres["makespan_s"]}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args)
@@ -355,6 +458,10 @@ def main():
                    "pairs_per_particle": res["pairs"] / max(res["n_active"], 1),
                    "candidates_per_particle": res["cand"] / max(res["n_active"], 1)},
         "roofline": roof,
+        "roofline_neighbours": nb_roof,
+        "whole_step": step,
+        "precision_pairs": precision_pairs,
+        "sweep": sweep_rec,
         "kernels": res["kernels"],
         "cpu_baseline": cpu,
     }

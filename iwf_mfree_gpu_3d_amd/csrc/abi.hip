@@ -225,6 +225,7 @@ struct mfg_grid {
   int *parts = nullptr;
   int nparts_cap = 0;
   double *bbox_part = nullptr;
+  std::vector<double> bbox_host;  // host copy of the bbox partials (sized at creation)
   bool scanned = false;  // offsets valid for the last hashed particle set
   GridInfo info() const {
     GridInfo g;
@@ -304,6 +305,7 @@ int grid_alloc(mfg_grid *g) {
   g->nparts_cap = (int)((g->max_cell + 4095) / 4096) + 1;
   HIP_TRY(hipMalloc(&g->parts, sizeof(int) * g->nparts_cap));
   HIP_TRY(hipMalloc(&g->bbox_part, sizeof(double) * 7 * 1024));
+  g->bbox_host.assign(7 * 1024, 0.);
   return MFG_OK;
 }
 
@@ -324,7 +326,7 @@ GridBuffers grid_buffers(mfg_grid *g, mfg_particles *p) {
 int grid_update_unlocked(mfg_grid *g, mfg_particles *p, double kernel_width) {
   const int nb = 1024;
   launch_bbox_reduce(g_lib.stream, p->st(), p->cnt, p->N_init, g->bbox_part, nb);
-  std::vector<double> part(7 * nb);
+  std::vector<double> &part = g->bbox_host;
   HIP_TRY(hipMemcpyAsync(part.data(), g->bbox_part, sizeof(double) * 7 * nb, hipMemcpyDeviceToHost,
                          g_lib.stream));
   HIP_TRY(hipStreamSynchronize(g_lib.stream));
@@ -430,9 +432,18 @@ int contact_search_buffers(mfg_particles *p, int n_tools, ContactSearch &cs) {
   return MFG_OK;
 }
 
-// neighbour lists (+ PSE) of the current records on grid g
-int run_neighbours(mfg_grid *g, mfg_particles *p, const Consts &c, int want_pse) {
+// neighbour lists (+ PSE) of the current records on grid g; stats: keep the per-workgroup
+// candidate / pair totals (a buffer sized on first use) for mfg_particle_gpu_pair_stats
+int run_neighbours(mfg_grid *g, mfg_particles *p, const Consts &c, int want_pse, bool stats = true) {
   const GridInfo gi = g->info();
+  if (!stats) {
+    p->stat_n = 0;
+    launch_neighbours(g_lib.stream, gi, g->count, p->nl4, p->pse2, p->st().h, p->st().flags, c,
+                      p->nbr, p->nl_stride, p->count, p->st().T_t, p->cnt, want_pse, nullptr,
+                      p->poison);
+    HIP_TRY(hipGetLastError());
+    return MFG_OK;
+  }
   const int nb = neighbour_blocks(gi);
   if (nb > p->stat_cap) {
     if (p->stat_part) (void)hipFree(p->stat_part);
@@ -460,6 +471,10 @@ int check_counters(mfg_particles *p) {
   if (p->cnt_h->err_grid) return fail(MFG_ERR_GRID, "unblanked particle outside the grid");
   if (p->cnt_h->err_list)
     return fail(MFG_ERR_STATE, "a neighbour list entry lies outside the particle's cell stencil");
+  // the reference stops the run here (src/grid_gpu_base.cu:159-163); N is 0 only when every
+  // particle was hard-blanked
+  if (p->N_host == 0 && p->N_init > 0)
+    return fail(MFG_ERR_ALL_BLANKED, "all particles left bbox due to blanking");
   return MFG_OK;
 }
 
@@ -898,6 +913,9 @@ int mfg_particle_gpu_counts(mfg_particles *p, unsigned *N, unsigned *N_init) {
 int mfg_particle_gpu_download(mfg_particles *p, int f, double *host) {
   NEED_LIB();
   if (!p || !host) return fail(MFG_ERR_ARG, "null argument");
+  // errors of the queued steps surface here too (the copies below synchronise anyway)
+  const int rc0 = check_counters(p);
+  if (rc0 != MFG_OK) return rc0;
   const unsigned N = p->N_init;
   const State &st = p->st();
   hipStream_t s = g_lib.stream;
@@ -1181,16 +1199,15 @@ int mfg_grid_assign_hashes(const mfg_grid *gc, mfg_particles *p) {
 int mfg_grid_adapt_particle_number(const mfg_grid *g, mfg_particles *p) {
   NEED_LIB();
   if (!g || !p) return fail(MFG_ERR_ARG, "null argument");
-  int rc = check_counters(p);
-  if (rc != MFG_OK) return rc;
-  if (p->N_host == 0) return fail(MFG_ERR_ALL_BLANKED, "all particles left bbox due to blanking");
-  return MFG_OK;
+  return check_counters(p);
 }
 
 int mfg_grid_sort(mfg_grid *g, mfg_particles *p) {
   NEED_LIB();
   if (!g || !p) return fail(MFG_ERR_ARG, "null argument");
   if (!g->scanned) sort_keys(g, p, nullptr);
+  const int rc = check_counters(p);
+  if (rc != MFG_OK) return rc;
   Consts c = make_consts(0., p->precision);
   permute(g, p, c, 0);
   // the keys now describe the sorted order; a further sort hashes again
@@ -1562,20 +1579,116 @@ int mfg_leap_frog_destroy(mfg_leap_frog *lf) {
   return MFG_OK;
 }
 
-int mfg_leap_frog_step(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int record,
-                       mfg_tool *const *tools, int n_tools, const mfg_blanking_state *blanking,
-                       mfg_tool_forces *forces, double dt, double t_current) {
+}  // extern "C"
+
+namespace {
+
+// Work memory of one step beyond the particle set, grid and tools.  A null `base` means the
+// step's own buffers, sized on first use (hipMalloc); otherwise the regions are carved out of
+// [base, base + bytes) and the step allocates nothing.
+struct StepWork {
+  unsigned char *base = nullptr;
+  size_t bytes = 0;
+};
+
+// suspends the per-kernel event timing for the scope (its events are created on demand)
+struct ProfOff {
+  bool saved;
+  ProfOff(bool on) : saved(g_lib.prof) {
+    if (on) g_lib.prof = false;
+  }
+  ~ProfOff() { g_lib.prof = saved; }
+};
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+int step_impl(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int record,
+              mfg_tool *const *tools, int n_tools, const mfg_blanking_state *blanking,
+              mfg_tool_forces *forces, double dt, double t_current, const StepWork &work) {
   NEED_LIB();
   if (!lf || !p || !g) return fail(MFG_ERR_ARG, "null argument");
   if (n_tools < 0 || n_tools > MAX_TOOLS || (n_tools > 0 && !tools))
     return fail(MFG_ERR_ARG, "%d tools (max %d)", n_tools, MAX_TOOLS);
   if (record && (!forces || forces->n < (unsigned)n_tools))
     return fail(MFG_ERR_ARG, "recording forces needs a tool_forces of %d tools", n_tools);
+  for (int k = 0; k < n_tools; k++)
+    if (!tools[k]) return fail(MFG_ERR_ARG, "tool %d is null", k);
+  const bool fixed_work = work.base != nullptr;
+  ProfOff prof_off(fixed_work);
   hipStream_t s = g_lib.stream;
   Consts c = make_consts(dt, p->precision);
   // spatial sorting
   int rc = grid_update_geometry(g, p, 2.0, t_current);
   if (rc != MFG_OK) return rc;
+  const GridInfo gi = g->info();
+  // visiting order of the list pair pass: strips of 8 grid columns for 192-byte FP64 records,
+  // 16 for 112-byte FP32 records (measured best on the 1M single-grain cut); negative: the
+  // sorted order.  Automatic: the sorted order when its window (the records of about five
+  // x-rows) fits in half an XCD's 4 MB L2, where strips gain nothing and cost two launches.
+  const int tiled = g_lib.opt.pair_pass == MFG_PAIR_TILED ? 1 : 0;
+  const double rec_b = p->precision == MFG_FP64 ? 192. : 112.;
+  const bool small = 5. * rec_b * p->N_init / (gi.nx > 0 ? gi.nx : 1) <= 2. * 1024 * 1024;
+  const int strip = g_lib.opt.pair_strip > 0    ? g_lib.opt.pair_strip
+                    : g_lib.opt.pair_strip < 0  ? 0
+                    : small                     ? 0
+                    : p->precision == MFG_FP64 ? 8
+                                               : 16;
+  const int nseg = (!tiled && strip > 0) ? pair_order_segments(gi, strip) : 0;
+  int n_exh = 0;
+  for (int k = 0; k < n_tools; k++) {
+    const mfg_tool *t = tools[k];
+    n_exh += (t->algo == MFG_CONTACT_EXHAUSTIVE && t->n_boxed > 0 && t_current >= t->birth &&
+              t_current <= t->death);
+  }
+  const bool search = n_exh > 0 && g_lib.opt_contact_search;
+  // work memory: the caller's region, or the step's own buffers
+  int *seg_pos = nullptr;
+  ContactSearch cs{};
+  if (fixed_work) {
+    // segments | list counter | slots (n_init per tool) | list + results (cap entries).  The
+    // list may hold fewer pairs than are in tool boxes: k_contact_bc searches the others per
+    // thread, which finds the same closest triangle.
+    size_t off = 0;
+    const size_t seg_b = align256(sizeof(int) * (size_t)nseg);
+    const size_t pairs = (size_t)p->N_init * (size_t)n_tools;
+    const size_t slot_b = search ? align256(sizeof(int) * pairs) : 0;
+    const size_t entry_b = sizeof(int2) + 6 * sizeof(double);
+    const size_t min_cap = search ? std::min<size_t>(pairs, 4096) : 0;
+    const size_t need = seg_b + (search ? 256 + slot_b + align256(entry_b * min_cap) : 0);
+    if (need > work.bytes)
+      return fail(MFG_ERR_ARG,
+                  "allocation-free step: %u particles x %d tools and %d visiting-order segments "
+                  "need %zu bytes of work memory, more than the %zu the library holds",
+                  p->N_init, n_tools, nseg, need, work.bytes);
+    if (nseg > 0) seg_pos = (int *)(work.base + off);
+    off += seg_b;
+    if (search) {
+      cs.list_n = (unsigned *)(work.base + off);
+      off += 256;
+      cs.slot = (int *)(work.base + off);
+      off += slot_b;
+      const size_t cap = std::min<size_t>(std::min<size_t>(pairs, (size_t)1 << 22),
+                                          (work.bytes - off - 256) / entry_b);
+      cs.cap = (unsigned)cap;
+      cs.list = (int2 *)(work.base + off);
+      off += align256(sizeof(int2) * cap);
+      cs.res = (double *)(work.base + off);
+    }
+  } else {
+    if (nseg > p->seg_cap) {
+      if (p->seg_pos) (void)hipFree(p->seg_pos);
+      p->seg_pos = nullptr;
+      p->seg_cap = 0;
+      if (hipMalloc(&p->seg_pos, sizeof(int) * (size_t)nseg) != hipSuccess)
+        return fail(MFG_ERR_NOMEM, "pair-pass order (%d segments)", nseg);
+      p->seg_cap = nseg;
+    }
+    seg_pos = p->seg_pos;
+    if (search) {
+      rc = contact_search_buffers(p, n_tools, cs);
+      if (rc != MFG_OK) return rc;
+    }
+  }
   g->scanned = false;
   sort_keys(g, p, blanking);
   permute(g, p, c, 1);
@@ -1583,37 +1696,14 @@ int mfg_leap_frog_step(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int rec
   const bool pse = c.heat == MFG_HEAT_PSE && c.pse_on;
   {
     ProfScope ps("neighbours");
-    rc = run_neighbours(g, p, c, pse ? 1 : 0);
+    rc = run_neighbours(g, p, c, pse ? 1 : 0, !fixed_work);
     if (rc != MFG_OK) return rc;
   }
   {
-    const int tiled = g_lib.opt.pair_pass == MFG_PAIR_TILED ? 1 : 0;
-    const GridInfo gi = g->info();
-    // strip width: 8 grid columns for 192-byte FP64 records, 16 for 112-byte FP32 records
-    // (measured best on the 1M single-grain cut); negative: the sorted order.  Automatic: the
-    // sorted order when its window (the records of about five x-rows) fits in half an XCD's
-    // 4 MB L2, where strips gain nothing and cost two launches per step.
-    const double rec_b = p->precision == MFG_FP64 ? 192. : 112.;
-    const bool small = 5. * rec_b * p->N_init / (gi.nx > 0 ? gi.nx : 1) <= 2. * 1024 * 1024;
-    const int strip = g_lib.opt.pair_strip > 0    ? g_lib.opt.pair_strip
-                      : g_lib.opt.pair_strip < 0  ? 0
-                      : small                     ? 0
-                      : p->precision == MFG_FP64 ? 8
-                                                 : 16;
     const int *trav = nullptr;
-    if (!tiled && strip > 0) {
+    if (nseg > 0) {
       ProfScope ps("pair_order");
-      const int nseg = pair_order_segments(gi, strip);
-      if (nseg > p->seg_cap) {
-        if (p->seg_pos) (void)hipFree(p->seg_pos);
-        p->seg_pos = nullptr;
-        p->seg_cap = 0;
-        if (hipMalloc(&p->seg_pos, sizeof(int) * (size_t)nseg) != hipSuccess)
-          return fail(MFG_ERR_NOMEM, "pair-pass order (%d segments)", nseg);
-        p->seg_cap = nseg;
-      }
-      launch_pair_order(s, p->N_init, gi, g->count, p->key_sorted, p->cnt, strip, p->seg_pos,
-                        p->trav);
+      launch_pair_order(s, p->N_init, gi, g->count, p->key_sorted, p->cnt, strip, seg_pos, p->trav);
       trav = p->trav;
     }
     ProfScope ps(p->precision == MFG_FP64          ? "pair_fp64"
@@ -1634,7 +1724,6 @@ int mfg_leap_frog_step(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int rec
   if (record) HIP_TRY(hipMemsetAsync(forces->d, 0, sizeof(float) * 3 * forces->n, s));
   for (int k = 0; k < n_tools; k++) {
     mfg_tool *t = tools[k];
-    if (!t) return fail(MFG_ERR_ARG, "tool %d is null", k);
     rc = mfg_tool_update(t, t_current, dt);
     if (rc != MFG_OK) return rc;
     ToolView &v = ts.t[k];
@@ -1661,13 +1750,8 @@ int mfg_leap_frog_step(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int rec
   }
   if (n_tools > 0)
     for (int a = 0; a < 3; a++) ts.fric_vel[a] = tools[0]->vel[a];
-  int n_exh = 0;
-  for (int k = 0; k < n_tools; k++) n_exh += (ts.t[k].algo == MFG_CONTACT_EXHAUSTIVE && ts.t[k].active);
-  if (n_exh > 0 && g_lib.opt_contact_search) {
+  if (search) {
     ProfScope ps("contact_search");
-    ContactSearch cs;
-    rc = contact_search_buffers(p, n_tools, cs);
-    if (rc != MFG_OK) return rc;
     for (int k = 0; k < n_tools; k++)
       if (ts.t[k].algo == MFG_CONTACT_EXHAUSTIVE) {
         ts.t[k].slot = cs.slot + (size_t)k * p->N_init;
@@ -1683,6 +1767,17 @@ int mfg_leap_frog_step(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int rec
   HIP_TRY(hipGetLastError());
   return MFG_OK;
 }
+
+}  // namespace
+
+extern "C" {
+
+int mfg_leap_frog_step(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int record,
+                       mfg_tool *const *tools, int n_tools, const mfg_blanking_state *blanking,
+                       mfg_tool_forces *forces, double dt, double t_current) {
+  return step_impl(lf, p, g, record, tools, n_tools, blanking, forces, dt, t_current, StepWork{});
+}
+
 
 // ---- profiling ----
 int mfg_profile_enable(int on) {

@@ -215,13 +215,15 @@ __global__ void __launch_bounds__(BS) k_fixup(const int *off, long num_cell, int
   }
 }
 
-// what the CSPM sum reads of a neighbour, 32 bytes: FP64 (x, y, z, vol), vol = -1 for a tool
-// particle (a volume is positive); FP32 the record's first 32 bytes (x, y, z, {tool, vol})
+// what the CSPM sum reads of a neighbour, 32 bytes: FP64 (x, y, z, vol), the volume of a tool
+// particle replaced by the NaN pattern CSPM_TOOL_BITS (no volume m / rho has that payload, so a
+// workpiece particle of negative or non-finite density still counts as the reference counts it);
+// FP32 the record's first 32 bytes (x, y, z, {tool, vol})
 template <class RecT>
 __device__ __forceinline__ double4 cspm_entry(const double *xh, double vol, bool tool);
 template <>
 __device__ __forceinline__ double4 cspm_entry<Rec64>(const double *xh, double vol, bool tool) {
-  return make_double4(xh[0], xh[1], xh[2], tool ? -1.0 : vol);
+  return make_double4(xh[0], xh[1], xh[2], tool ? __longlong_as_double(CSPM_TOOL_BITS) : vol);
 }
 template <>
 __device__ __forceinline__ double4 cspm_entry<Rec32>(const double *xh, double vol, bool tool) {

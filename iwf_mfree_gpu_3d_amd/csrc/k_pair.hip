@@ -425,13 +425,14 @@ __global__ void __launch_bounds__(NB_BLOCK)
     cell_lists<PSE_MODE>(s, e, T, lane, s_pd[w], fetch, nl4, pse2, h, flags, c, nbr, cap, count,
                          T_t, cnt, poison, s_exp, n_cand, n_pair);
   }
-  // candidate / pair totals of this workgroup (work accounting of the roofline figures)
+  // candidate / pair totals of this workgroup (work accounting of the roofline figures); a null
+  // stat_part skips them
   if (lane == 0) {
     s_stat[w][0] = n_cand;
     s_stat[w][1] = n_pair;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0 && stat_part) {
     unsigned long long a = 0, bb = 0;
 #pragma unroll
     for (int k = 0; k < NB_WAVES; k++) {
@@ -776,9 +777,10 @@ struct Pair64 {
     if (g.q.w == 1.) return;
     cspm_add(c, g.p.x, g.p.y, g.p.z, g.q.y);
   }
-  // the same term from the compact entry (x, y, z, vol), vol < 0 marking a tool particle
+  // the same term from the compact entry (x, y, z, vol), the CSPM_TOOL_BITS pattern marking a
+  // tool particle (tool_particle == 1, src/interactions_gpu.cu:251-254)
   __device__ __forceinline__ void cspm_term(const Consts &c, const double4 &g) {
-    if (g.w < 0.) return;
+    if (__double_as_longlong(g.w) == CSPM_TOOL_BITS) return;
     cspm_add(c, g.x, g.y, g.z, g.w);
   }
   __device__ __forceinline__ void cspm_add(const Consts &c, double xj, double yj, double zj,

@@ -8,7 +8,7 @@
 //   half-step records (sorted order, rebuilt every step by the permute/predict kernel):
 //     nl4   : double4 (x, y, z, blanked)          -- candidate filter of the neighbour-list pass
 //     pse2  : double2 (T, rho)                    -- PSE heat pass (only with PSE)
-//     cs4   : double4 (x, y, z, vol or -1 for a tool particle) FP64; the first 32 bytes of
+//     cs4   : double4 (x, y, z, vol; CSPM_TOOL_BITS for a tool particle) FP64; the first 32 bytes of
 //             the record FP32                     -- CSPM sum of the pair pass
 //     rec   : pair record, 192 B (FP64) / 112 B (FP32) -- everything a neighbour contributes
 //   neighbour list: int32 nbr[i * cap + k] (row-major), count[i]
@@ -35,6 +35,9 @@ constexpr uint32_t FL_FIXED_ONE = 8u;  // fixed == 1        (src/actions_gpu.cu:
 constexpr uint32_t FL_TOOL_NZ = 16u;   // tool_particle != 0 (src/interactions_gpu.cu:167,230)
 
 constexpr int KEY_BLANK = 0x7fffffff;  // INT_MAX, hash of a hard-blanked particle
+// volume slot of a tool particle's FP64 CSPM entry: a quiet NaN with a payload that no
+// arithmetic result carries (0/0 and inf/inf give the payload-free default NaN)
+constexpr long long CSPM_TOOL_BITS = 0x7ff80000cafe7001ll;
 
 struct State {
   double *x, *y, *z;

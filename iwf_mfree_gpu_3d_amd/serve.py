@@ -97,7 +97,8 @@ class Instance:
 
     DX = 5e-4  # lattice spacing [cm]
 
-    def __init__(self, req: Request, stream: int = 0, choice: tuple = (api.MFG_PAIR_LIST, 0)):
+    def __init__(self, req: Request, stream: int = 0, choice: tuple = (api.MFG_PAIR_LIST, 0),
+                 max_neighbours: int = 0):
         self.req = req
         self.stream = stream
         self.choice = tuple(choice)  # (pair pass, strip) the instance's steps run with
@@ -109,7 +110,8 @@ class Instance:
                                   speed=req.speed, rake_deg=req.rake_deg, jc_A=cfg.jc_A * req.jc_scale[0],
                                   jc_B=cfg.jc_B * req.jc_scale[1], jc_C=cfg.jc_C * req.jc_scale[2],
                                   steps=req.steps)
-        self.particles, self.grid = scenes.setup_cut(self.scene, precision=req.precision, record_forces=False)
+        self.particles, self.grid = scenes.setup_cut(self.scene, precision=req.precision, record_forces=False,
+                                                     max_neighbours=max_neighbours)
         self.tools = api.global_tool
         self.blanking = api.global_blanking
         self.dt = api.global_time_dt
@@ -158,10 +160,18 @@ class Instance:
     def done(self) -> bool:
         return self.k >= self.req.steps
 
-    def finish(self) -> None:
-        """Waits for the instance's queued work (its stream selected for what follows)."""
+    def finish(self) -> int:
+        """Waits for the instance's queued work (its stream selected for what follows) and reads
+        its device error counters, which keep the first failure of any step: MFG_OK, or the code
+        of a neighbour-list overflow (MFG_ERR_CAPACITY), an unblanked particle outside the grid
+        (MFG_ERR_GRID) or every particle blanked (MFG_ERR_ALL_BLANKED)."""
         api.select_stream(self.stream)
         api.synchronize()
+        try:
+            self.particles.counts()
+        except _lib.MfgError as e:
+            return e.code
+        return _lib.MFG_OK
 
     def release(self) -> None:
         api.select_stream(self.stream)
@@ -169,9 +179,13 @@ class Instance:
 
 
 def serve(requests: list, hbm_budget: float = 250e9, steps_per_round: int = 5, on_done=None,
-          clock=time.perf_counter, log=None, streams: int = 4, tuning=None) -> dict:
+          clock=time.perf_counter, log=None, streams: int = 4, tuning=None, max_neighbours: int = 0) -> dict:
     """Runs the request loop until every request has finished.  `on_done(instance)` is called
-    for each finished instance before its state is released.  `streams`: size of the stream pool
+    for each instance that finished without an error, before its state is released; a request
+    whose instance reports a device error gets that code as its record's "error" and no time to
+    result (the error counters are sticky, so the check at the end covers every step).
+    `max_neighbours`: neighbour-list capacity of the instances (0: the library default).
+    `streams`: size of the stream pool
     the instances are spread over.  `tuning`: an autotune.Table whose per-bucket pair-pass
     choice each instance runs with (None: the library defaults).  Returns per-request records
     and p50 / p99 time to result and the aggregate particle-steps/s."""
@@ -195,7 +209,7 @@ def serve(requests: list, hbm_budget: float = 250e9, steps_per_round: int = 5, o
             load[k] += 1
             nx, ny, nz = r.lattice()
             choice = tuning.get(nx * ny * nz, r.precision) if tuning is not None else (api.MFG_PAIR_LIST, 0)
-            inst = Instance(r, k, choice)
+            inst = Instance(r, k, choice, max_neighbours=max_neighbours)
             used += need
             active.append(inst)
             records[r.rid] = {"rid": r.rid, "arrival": r.arrival, "admitted": clock() - t0, "particles": inst.n,
@@ -210,12 +224,17 @@ def serve(requests: list, hbm_budget: float = 250e9, steps_per_round: int = 5, o
         for inst in list(active):
             steps_total += inst.n * inst.advance(steps_per_round)
             if inst.done:
-                inst.finish()
+                err = inst.finish()
                 rec = records[inst.req.rid]
                 rec["finished"] = clock() - t0
-                rec["time_to_result"] = rec["finished"] - rec["arrival"]
-                if on_done:
-                    on_done(inst)
+                if err in (None, _lib.MFG_OK):
+                    rec["time_to_result"] = rec["finished"] - rec["arrival"]
+                    if on_done:
+                        on_done(inst)
+                else:
+                    rec["error"] = int(err)
+                    if log:
+                        log(f"t={rec['finished']:8.2f}s request {inst.req.rid} failed with error {err}")
                 inst.release()
                 load[inst.stream] -= 1
                 active.remove(inst)
@@ -223,7 +242,7 @@ def serve(requests: list, hbm_budget: float = 250e9, steps_per_round: int = 5, o
     api.device_synchronize()
     api.select_stream(0)
     makespan = clock() - t0
-    ttr = np.array([r["time_to_result"] for r in records.values()])
+    ttr = np.array([r["time_to_result"] for r in records.values() if "time_to_result" in r])
     return {"requests": [records[k] for k in sorted(records)],
             "p50_s": float(np.percentile(ttr, 50)) if ttr.size else 0.0,
             "p99_s": float(np.percentile(ttr, 99)) if ttr.size else 0.0,

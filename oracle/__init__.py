@@ -37,14 +37,46 @@ def build(force: bool = False) -> str:
     return _LIB
 
 
+def build_native() -> str:
+    """The CPU-baseline build of the same source: g++ -O3 -march=native with OpenMP, still without
+    FP contraction (so its results are the -O2 build's), compiled for the host it runs on into
+    oracle/_native/, one library per host CPU model."""
+    import hashlib
+    import platform
+    model = platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    tag = hashlib.sha1(model.encode()).hexdigest()[:10]
+    out_dir = os.path.join(_HERE, "_native")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, f"liboracle_native_{tag}.so")
+    if not os.path.exists(out) or os.path.getmtime(out) < os.path.getmtime(_SRC):
+        cmd = ["g++", "-O3", "-march=native", "-std=c++17", "-fopenmp", "-ffp-contract=off", "-fPIC",
+               "-shared", "-o", out + ".tmp", _SRC]
+        subprocess.run(cmd, check=True)
+        os.replace(out + ".tmp", out)
+    return out
+
+
 _lib = None
 
 
 def lib():
+    """The oracle library: the -O2 build, or the library named by MFG_ORACLE_LIB (the CPU
+    baseline's native build)."""
     global _lib
     if _lib is None:
-        build()
-        _lib = C.CDLL(_LIB)
+        path = os.environ.get("MFG_ORACLE_LIB")
+        if not path:
+            build()
+            path = _LIB
+        _lib = C.CDLL(path)
         _declare(_lib)
     return _lib
 
@@ -96,6 +128,7 @@ def _declare(L):
         "o_tool_cells": (None, [_vp, C.c_int, _up]),
         "o_tool_tris": (None, [_vp, C.c_int, C.c_int, _dp]),
         "o_tool_inside": (None, [_vp, C.c_int, C.c_int, _dp, _ip, _ip]),
+        "o_tool_closest": (None, [_vp, C.c_int, C.c_int, _dp, _dp, _dp]),
         "o_set_time": (None, [_vp, C.c_double, C.c_double]),
         "o_set_threads": (None, [_vp, C.c_int]),
         "o_step": (C.c_int, [_vp, C.c_int]),
@@ -306,6 +339,10 @@ class OracleSim:
         a, b = _keep(bbmin, bbmax)
         lib().o_set_grid_locked(self._h, _d(a), _d(b), float(h))
 
+    def set_grid_unlocked(self, max_cell):
+        """Geometry follows the particles (grid_gpu_green(max_cell, num_part))."""
+        lib().o_set_grid_unlocked(self._h, int(max_cell))
+
     def set_grid_params(self, vel_bbox=(0, 0, 0), min_fixed=False, max_fixed=False, hard_blank=True):
         (v,) = _keep(vel_bbox)
         lib().o_set_grid_params(self._h, _d(v), int(min_fixed), int(max_fixed), int(hard_blank))
@@ -358,6 +395,14 @@ class OracleSim:
         b = np.zeros(p.shape[0], dtype=np.int32)
         lib().o_tool_inside(self._h, k, p.shape[0], _d(p), a.ctypes.data_as(_ip), b.ctypes.data_as(_ip))
         return a.astype(bool), b.astype(bool)
+
+    def tool_closest(self, k, pts):
+        """Closest point and normal the hashed algorithm's shell search finds for each point."""
+        p = np.ascontiguousarray(pts, dtype=np.float64)
+        cp = np.zeros_like(p)
+        nn = np.zeros_like(p)
+        lib().o_tool_closest(self._h, k, p.shape[0], _d(p), _d(cp), _d(nn))
+        return cp, nn
 
     def set_time(self, dt, t):
         lib().o_set_time(self._h, float(dt), float(t))

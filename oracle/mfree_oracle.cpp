@@ -27,6 +27,7 @@
 // Compile with -ffp-contract=off so that every expression is evaluated exactly as written.
 
 #include <algorithm>
+#include <parallel/algorithm>
 #include <cfloat>
 #include <climits>
 #include <cmath>
@@ -715,6 +716,12 @@ static bool contact_by_normal(const tool &T, const double gbbmin[3], v3 qp, v3 &
     ix = (int)a; iy = (int)b; iz = (int)c;
   }
   int nx = (int)T.gnx, ny = (int)T.gny, nz = (int)T.gnz;
+  // a point on the grid's upper faces (the constructor's outside point bbmax + 1e-6, or a
+  // particle at the edge of the moving box) can index one cell past the grid, where the
+  // reference reads past its cell table; the index is clamped to the last cell, as the device does
+  ix = std::min(ix, nx - 1);
+  iy = std::min(iy, ny - 1);
+  iz = std::min(iz, nz - 1);
   int box = ix * ny * nz + iy * nz + iz;
   double min_dist = DBL_MAX;
   v3 best_cp, best_n;
@@ -770,6 +777,9 @@ static bool contact_by_ray(const tool &T, const double gbbmin[3], v3 qp) {
   int iy = (int)(unsigned)((qp.y - gbbmin[1]) / T.gdx);
   int iz = (int)(unsigned)((qp.z - gbbmin[2]) / T.gdx);
   int ny = (int)T.gny, nz = (int)T.gnz;
+  ix = std::min(ix, (int)T.gnx - 1);  // clamped as in contact_by_normal
+  iy = std::min(iy, ny - 1);
+  iz = std::min(iz, nz - 1);
   v3 dir(0., 0., 1.);
   unsigned n_inter = 0;
   for (; iz < nz; ++iz) {
@@ -966,8 +976,9 @@ static void assign_hashes(sim &s) {
 
 static void adapt_particle_number(sim &s) {
   if (!s.g_hard_blank) return;
-  unsigned num_out = 0;
-  for (unsigned i = 0; i < s.N_init; i++)
+  long num_out = 0;
+#pragma omp parallel for reduction(+ : num_out)
+  for (long i = 0; i < (long)s.N_init; i++)
     if (s.hash[i] == INT_MAX) num_out++;
   s.N = s.N_init - num_out;
   if (s.N == 0) s.status = 1;
@@ -976,7 +987,8 @@ static void adapt_particle_number(sim &s) {
 template <class T>
 static void permute(std::vector<T> &v, const std::vector<int> &idx, unsigned n) {
   std::vector<T> tmp(v);
-  for (unsigned i = 0; i < n; i++) v[i] = tmp[idx[i]];
+#pragma omp parallel for
+  for (long i = 0; i < (long)n; i++) v[i] = tmp[idx[i]];
 }
 
 // Green sort (src/grid_gpu_green.cu:265-301): stable sort_by_key over N_init, reorder the 19
@@ -985,10 +997,13 @@ static void grid_sort(sim &s) {
   unsigned n = s.N_init;
   std::vector<int> order(n);
   for (unsigned i = 0; i < n; i++) order[i] = (int)i;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int a, int b) { return s.hash[a] < s.hash[b]; });
+  // ordered by (key, original index): a total order, so any sort gives the stable sort by key
+  __gnu_parallel::sort(order.begin(), order.end(), [&](int a, int b) {
+    return s.hash[a] < s.hash[b] || (s.hash[a] == s.hash[b] && a < b);
+  });
   std::vector<int> keys(n);
-  for (unsigned i = 0; i < n; i++) keys[i] = s.hash[order[i]];
+#pragma omp parallel for
+  for (long i = 0; i < (long)n; i++) keys[i] = s.hash[order[i]];
   s.hash = keys;
   s.idx = order;
   std::fill(s.cell_start.begin(), s.cell_start.end(), -1);
@@ -1029,7 +1044,8 @@ static void grid_sort(sim &s) {
 
 // -- leap frog (src/leap_frog.cu:30-100) ------------------------------------------------------
 static void lf_init(sim &s) {
-  for (unsigned i = 0; i < s.N_init; i++) {
+#pragma omp parallel for
+  for (long i = 0; i < (long)s.N_init; i++) {
     if (s.blanked[i] == 1.) continue;
     s.pos0[i] = s.pos[i]; s.vel0[i] = s.vel[i]; s.S0[i] = s.S[i];
     s.rho0[i] = s.rho[i]; s.T0[i] = s.T[i];
@@ -1057,7 +1073,8 @@ static void lf_advance(sim &s, double f) {
 static void material_eos(sim &s) {
   double rho0 = s.a_phys.rho0;
   double c0 = sqrt_site(s.a_phys.K / rho0, s.single_sqrt);
-  for (unsigned i = 0; i < s.N; i++) {
+#pragma omp parallel for
+  for (long i = 0; i < (long)s.N; i++) {
     if (s.blanked[i] == 1. || s.tool_p[i] == 1.) continue;
     s.p[i] = c0 * c0 * (s.rho[i] - rho0);
   }
@@ -1283,8 +1300,9 @@ static void rate_actions(sim &s) {
 }
 
 static void invalidate_rates(sim &s) {
-  for (unsigned i = 0; i < s.N; i++)
-    if (rates_invalid(s, i)) s.blanked[i] = 1.;
+#pragma omp parallel for
+  for (long i = 0; i < (long)s.N; i++)
+    if (rates_invalid(s, (unsigned)i)) s.blanked[i] = 1.;
 }
 
 // radial return (src/actions_gpu.cu:371-449)
@@ -1435,7 +1453,8 @@ static void contact_force(sim &s, tool &T, int tool_idx, bool record) {
 static void fric_heat_gen(sim &s, const double vt[3]) {
   if (!s.a_fric) return;
   double eta = s.a_twp.eta;
-  for (unsigned i = 0; i < s.N; i++) {
+#pragma omp parallel for
+  for (long i = 0; i < (long)s.N; i++) {
     if (s.blanked[i] == 1. || s.tool_p[i] == 1.) continue;
     v3 f = s.ft[i];
     double fmag = sqrt_site(f.x * f.x + f.y * f.y + f.z * f.z, s.single_sqrt);
@@ -1450,7 +1469,8 @@ static void fric_heat_gen(sim &s, const double vt[3]) {
 }
 
 static void boundary_conditions(sim &s) {
-  for (unsigned i = 0; i < s.N; i++) {
+#pragma omp parallel for
+  for (long i = 0; i < (long)s.N; i++) {
     if (s.blanked[i] == 1. || s.tool_p[i] == 1.) continue;
     if (s.fixed[i] != 0.) {
       s.vel[i] = s.vel_bc[i];
@@ -1463,12 +1483,14 @@ static void boundary_conditions(sim &s) {
 
 static void boundary_conditions_thermal(sim &s) {
   if (!s.a_thermal) return;
-  for (unsigned i = 0; i < s.N; i++)
+#pragma omp parallel for
+  for (long i = 0; i < (long)s.N; i++)
     if (s.fixed[i] == 1.) s.T[i] = s.a_twp.T_init;
 }
 
 static void move_tool_particles(sim &s, const double v[3]) {
-  for (unsigned i = 0; i < s.N_init; i++) {
+#pragma omp parallel for
+  for (long i = 0; i < (long)s.N_init; i++) {
     if (s.tool_p[i] != 1.) continue;
     s.pos[i].x = s.pos[i].x + v[0] * s.dt;
     s.pos[i].y = s.pos[i].y + v[1] * s.dt;
@@ -1498,7 +1520,8 @@ static void step(sim &s, bool record) {
   invalidate_rates(s);
   lf_advance(s, s.dt);
   plasticity_johnson_cook(s);
-  for (unsigned i = 0; i < s.N; i++) {
+#pragma omp parallel for
+  for (long i = 0; i < (long)s.N; i++) {
     s.fc[i] = v3(0., 0., 0.);
     s.ft[i] = v3(0., 0., 0.);
     s.nrm[i] = v3(0., 0., 0.);
@@ -1765,6 +1788,19 @@ void o_tool_inside(void *p, int tool_idx, int n, const double *pts, int *by_ray,
     v3 cp, nn;
     by_ray[i] = contact_by_ray(T, T.bbmin, q) ? 1 : 0;
     by_normal[i] = contact_by_normal(T, T.bbmin, q, cp, nn, true) ? 1 : 0;
+  }
+}
+
+// closest point and normal that the hashed algorithm's shell search returns for each point
+void o_tool_closest(void *p, int tool_idx, int n, const double *pts, double *cp_out, double *n_out) {
+  sim *s = (sim *)p;
+  const tool &T = s->tools[tool_idx];
+  for (int i = 0; i < n; i++) {
+    v3 q(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+    v3 cp, nn;
+    contact_by_normal(T, T.bbmin, q, cp, nn, true);
+    cp_out[3 * i] = cp.x; cp_out[3 * i + 1] = cp.y; cp_out[3 * i + 2] = cp.z;
+    n_out[3 * i] = nn.x; n_out[3 * i + 1] = nn.y; n_out[3 * i + 2] = nn.z;
   }
 }
 

@@ -1,6 +1,8 @@
 """The reference's run-time self-checks on the device: the NaN/Inf rate guard that blanks a
 particle (src/actions_gpu.cu:302-331) against the oracle, the neighbour-list capacity check, and
 the all-blanked stop (src/grid_gpu_base.cu:159-163)."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -67,8 +69,10 @@ def test_all_blanked_is_reported(hip_device):
     api.global_blanking = api.blanking(far, far + 0.1, max_vel=sc.params.max_vel)
     api.global_time_current = sc.params.dt
     lf = api.leap_frog(sc.n, grid.num_cell())
+    scenes.run(lf, parts, grid, 1)
     with pytest.raises(MfgError) as e:
-        scenes.run(lf, parts, grid, 1)
         grid.adapt_particle_number(parts)
     assert e.value.code == _lib.MFG_ERR_ALL_BLANKED
-    assert parts.N == 0
+    n, ni = C.c_uint(99), C.c_uint(0)
+    assert _lib.load().mfg_particle_gpu_counts(parts._h, C.byref(n), C.byref(ni)) == _lib.MFG_ERR_ALL_BLANKED
+    assert n.value == 0

@@ -34,7 +34,7 @@ class FakeInstance:
     peak = 0
     streams = []  # stream of each instance at admission, with the instances then active on it
 
-    def __init__(self, req, stream=0, choice=None):
+    def __init__(self, req, stream=0, choice=None, max_neighbours=0):
         self.req, self.k, self.n, self.stream, self.choice = req, 0, 1000, stream, choice
         FakeInstance.active += 1
         FakeInstance.peak = max(FakeInstance.peak, FakeInstance.active)
@@ -109,6 +109,22 @@ def test_result_same_alone_and_in_the_mix(hip_device):
     for f in FIELDS:
         np.testing.assert_array_equal(mix[0][f], alone[0][f], err_msg=f"field {f}")
     assert np.abs(alone[0][_lib.F_VEL]).max() > 0.0
+
+
+@pytest.mark.gpu
+def test_served_request_reports_device_errors(hip_device):
+    """A request whose neighbour lists overflow (32 entries against about 100 neighbours) comes
+    back with MFG_ERR_CAPACITY in its record, gets no time to result and is not handed to
+    on_done; a request in the same mix with full-size lists finishes normally."""
+    reqs = serve.sweep_requests(count=1, batched=True, n_min=4e3, n_max=6e3, steps=(6, 6), seed=21,
+                                fp32_share=0.0)
+    done = []
+    res = serve.serve(reqs, on_done=lambda inst: done.append(inst.req.rid), steps_per_round=3,
+                      max_neighbours=32)
+    rec = res["requests"][0]
+    assert rec.get("error") == _lib.MFG_ERR_CAPACITY and "time_to_result" not in rec and not done
+    res = serve.serve(reqs, on_done=lambda inst: done.append(inst.req.rid), steps_per_round=3)
+    assert "error" not in res["requests"][0] and done == [0]
 
 
 def test_autotune_picks_the_fastest_per_bucket(tmp_path):
