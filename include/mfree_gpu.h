@@ -268,6 +268,21 @@ int mfg_leap_frog_destroy(mfg_leap_frog *lf);
 int mfg_leap_frog_step(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int record_forces,
                        mfg_tool *const *tools, int n_tools, const mfg_blanking_state *blanking,
                        mfg_tool_forces *forces, double dt, double t_current);
+/* No counterpart in the reference.  mfg_leap_frog_step for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * same arguments, same results, and no hipMalloc / hipFree or allocating library call.  The
+ * step's work memory beyond the particle set, grid and tools (the visiting-order segments of the
+ * pair pass and the exhaustive contact search's slots, list and results) lives in one static
+ * device array of MFG_STEP_SCRATCH_BYTES per GPU, held until the process ends.  The slots take
+ * 4 bytes per particle and exhaustive tool, so N_init x tools up to about 16M fit; a step whose
+ * work memory does not fit returns MFG_ERR_ARG before any kernel that changes the particles runs.  Every call on one GPU shares
+ * the array: calls that overlap on one GPU (on different streams) race.  Per-kernel profiling
+ * is off inside the call. */
+#define MFG_STEP_SCRATCH_BYTES (64u << 20)
+int mfg_leap_frog_step_noalloc(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g,
+                               int record_forces, mfg_tool *const *tools, int n_tools,
+                               const mfg_blanking_state *blanking, mfg_tool_forces *forces,
+                               double dt, double t_current);
 /* debug_check_valid_full (src/actions_gpu.cu:739-745): MFG_ERR_NONFINITE if an unblanked
  * particle holds a non-finite value */
 int mfg_debug_check_valid(mfg_particles *p);

@@ -172,6 +172,8 @@ SIGNATURES = {
     "mfg_leap_frog_destroy": (_i, [_vp]),
     "mfg_leap_frog_step": (_i, [_vp, _vp, _vp, _i, C.POINTER(_vp), _i, C.POINTER(blanking_state), _vp,
                                 _d, _d]),
+    "mfg_leap_frog_step_noalloc": (_i, [_vp, _vp, _vp, _i, C.POINTER(_vp), _i, C.POINTER(blanking_state),
+                                        _vp, _d, _d]),
     "mfg_debug_check_valid": (_i, [_vp]),
     "mfg_profile_enable": (_i, [_i]),
     "mfg_profile_reset": (_i, []),

@@ -483,8 +483,11 @@ class leap_frog:
             _L().mfg_leap_frog_destroy(h)
             self._h = None
 
-    def step(self, particles: particle_gpu, g: grid_base, record_forces: bool = False) -> None:
-        """One time step with the process globals, as leap_frog::step (src/leap_frog.cu:121-195)."""
+    def step(self, particles: particle_gpu, g: grid_base, record_forces: bool = False,
+             allocation_free: bool = False) -> None:
+        """One time step with the process globals, as leap_frog::step (src/leap_frog.cu:121-195).
+        `allocation_free`: through mfg_leap_frog_step_noalloc (same results, no device allocation;
+        its work memory is the library's fixed per-GPU array)."""
         tools = global_tool
         arr = (C.c_void_p * max(len(tools), 1))(*[t._h for t in tools])
         bstate = None
@@ -492,9 +495,9 @@ class leap_frog:
             global_blanking.update()
             bstate = global_blanking.state()
         forces = global_tool_forces._h if (record_forces and global_tool_forces is not None) else None
-        check(_L().mfg_leap_frog_step(self._h, particles._h, g._h, int(bool(record_forces and forces)), arr,
-                                      len(tools), C.byref(bstate) if bstate is not None else None, forces,
-                                      global_time_dt, global_time_current))
+        fn = _L().mfg_leap_frog_step_noalloc if allocation_free else _L().mfg_leap_frog_step
+        check(fn(self._h, particles._h, g._h, int(bool(record_forces and forces)), arr, len(tools),
+                 C.byref(bstate) if bstate is not None else None, forces, global_time_dt, global_time_current))
 
 
 # ---- profiling ---------------------------------------------------------------------------------

@@ -72,6 +72,13 @@ struct Lib {
 Lib g_lib;
 thread_local std::string g_err;
 
+// Work memory of the allocation-free step (mfg_leap_frog_step_noalloc): one array per GPU, held
+// until the process ends.  Host code never passes `scratch` itself to a kernel or copy (in host
+// code the name is a stand-in, not a device address); it uses the device address that mfg_init
+// resolves with the library's device current (g_scratch).
+static __device__ __align__(256) unsigned char scratch[MFG_STEP_SCRATCH_BYTES];
+unsigned char *g_scratch = nullptr;
+
 int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 int fail(int code, const char *fmt, ...) {
   char buf[512];
@@ -635,6 +642,12 @@ int mfg_init(int device) {
   for (int k = 1; k < MFG_EARLY_STREAMS; k++)
     HIP_TRY(hipStreamCreateWithFlags(&g_lib.streams[k], hipStreamNonBlocking));
   g_lib.n_created = MFG_EARLY_STREAMS;
+  {
+    void *ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(MFG_ERR_HIP, "no device address for the step's work memory");
+    g_scratch = (unsigned char *)ptr;
+  }
   const char *lane_contact = std::getenv("MFG_LANE_CONTACT");
   g_lib.opt_contact_search = !(lane_contact && lane_contact[0] == '0');
   g_lib.device = device;
@@ -1721,7 +1734,7 @@ int step_impl(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int record,
   std::memset(&ts, 0, sizeof(ts));
   ts.n = n_tools;
   ts.record = record ? 1 : 0;
-  if (record) HIP_TRY(hipMemsetAsync(forces->d, 0, sizeof(float) * 3 * forces->n, s));
+  if (record) launch_zero_f32(s, forces->d, 3 * forces->n);
   for (int k = 0; k < n_tools; k++) {
     mfg_tool *t = tools[k];
     rc = mfg_tool_update(t, t_current, dt);
@@ -1776,6 +1789,18 @@ int mfg_leap_frog_step(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int rec
                        mfg_tool *const *tools, int n_tools, const mfg_blanking_state *blanking,
                        mfg_tool_forces *forces, double dt, double t_current) {
   return step_impl(lf, p, g, record, tools, n_tools, blanking, forces, dt, t_current, StepWork{});
+}
+
+int mfg_leap_frog_step_noalloc(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int record,
+                               mfg_tool *const *tools, int n_tools,
+                               const mfg_blanking_state *blanking, mfg_tool_forces *forces,
+                               double dt, double t_current) {
+  NEED_LIB();
+  if (!g_scratch) return fail(MFG_ERR_STATE, "the step's work memory has no device address");
+  StepWork w;
+  w.base = g_scratch;
+  w.bytes = MFG_STEP_SCRATCH_BYTES;
+  return step_impl(lf, p, g, record, tools, n_tools, blanking, forces, dt, t_current, w);
 }
 
 

@@ -806,6 +806,17 @@ void launch_force_reduce(hipStream_t s, const float *partials, int nblocks, int 
   hipLaunchKernelGGL(k_force_reduce, dim3(1), dim3(64), 0, s, partials, nblocks, n_tools, forces);
 }
 
+// clears n floats with a kernel (a memset node in a captured graph is what this avoids)
+__global__ void k_zero_f32(float *a, unsigned n) {
+  const unsigned i = blockIdx.x * BST + threadIdx.x;
+  if (i < n) a[i] = 0.f;
+}
+
+void launch_zero_f32(hipStream_t s, float *a, unsigned n) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_zero_f32, dim3(nbk(n)), dim3(BST), 0, s, a, n);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Warm start: uniformly weighted SPH interpolation of a scalar carried by samples (x, y, z, v)
 // onto points (src/Benchmarks/interp_utils.cpp:65-99): v(p) = sum_j weights W(|p - x_j|, h) v_j

@@ -231,6 +231,8 @@ void launch_contact_bc(hipStream_t s, const State &st, unsigned n_init, const To
                        const Consts &c, const Counters *cnt, float *partials, int nblocks);
 void launch_force_reduce(hipStream_t s, const float *partials, int nblocks, int n_tools,
                          float *forces);
+// forces[0 .. n) = 0 by a kernel
+void launch_zero_f32(hipStream_t s, float *a, unsigned n);
 
 // triangle binning on the GPU (grid_cpu_tri::get_cells)
 int tool_bin_triangles(hipStream_t s, const Tri *tris, int n_tris, const double bbmin[3],
