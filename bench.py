@@ -316,8 +316,9 @@ def sweep(args) -> None:
         tuning = autotune.tune(reqs, log=log)  # before the timed request loop
         tune_s = time.perf_counter() - t0
         log(f"autotune: {tune_s:.1f} s")
+    mode = "streams" if (args.autotune or args.mode == "streams") else "batched"
     res = serve.serve(reqs, hbm_budget=args.hbm_budget_gb * 1e9, log=log, streams=args.streams,
-                      tuning=tuning)
+                      tuning=tuning, mode=mode, use_graph=not args.no_graph)
     log(f"p50 {res['p50_s']:.2f} s, p99 {res['p99_s']:.2f} s, {res['particle_steps_per_s']:.4e} particle-steps/s "
         f"over {res['makespan_s']:.1f} s")
     out = {
@@ -327,7 +328,8 @@ def sweep(args) -> None:
         "scaling": "weak", "vs_baseline": None, "dtype": "mixed" if args.served > 0 else "f32",
         "data": "synthetic (seeded lattices + jitter, wedge tools)",
         "config": {"workload": workload, "requests": len(reqs), "hbm_budget_gb": args.hbm_budget_gb,
-                   "streams": args.streams, "autotune": bool(args.autotune)},
+                   "streams": args.streams, "autotune": bool(args.autotune), "mode": mode,
+                   "graph": mode == "batched" and not args.no_graph},
         "sweep": {"p50_time_to_result_s": res["p50_s"], "p99_time_to_result_s": res["p99_s"],
                   "makespan_s": res["makespan_s"], "particle_steps": res["particle_steps"]},
     }
@@ -375,6 +377,13 @@ def main():
                          "bucket by measurement first (outside the timed loop)")
     ap.add_argument("--streams", type=int, default=4,
                     help="served / batched forms: streams the instances are spread over")
+    ap.add_argument("--mode", choices=["batched", "streams"], default="batched",
+                    help="served / batched forms: one launch per kernel over the instances of a precision "
+                         "(batched) or each instance's own kernels on a stream pool (streams)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batched mode: launch every step's kernels instead of replaying a captured step")
+    ap.add_argument("--batched-instances", type=int, default=256,
+                    help="instances of the batched record in the default line (0: skip)")
     ap.add_argument("--error-report", action="store_true",
                     help="precision form instead: run the workload --steps steps in FP64 and in both FP32 "
                          "modes and report the FP32 L2 and max errors against the FP64 run")
@@ -402,7 +411,7 @@ def main():
     for k, v in sorted(res["kernels"].items(), key=lambda kv: -kv[1]["total_ms"]):
         log(f"  {k:20s} {v['launches']:6d} launches  avg {v['avg_ms']:.4f} ms  total {v['total_ms']:.1f} ms")
     log(f"GPU: {res['value']:.4e} particle-steps/s over {args.steps} steps ({res['secs']:.3f} s)")
-    precision_pairs, sweep_rec = None, None
+    precision_pairs, sweep_rec, batched_rec = None, None, None
     if not args.quick and args.preset == 2 and args.precision == "fp64":
         from iwf_mfree_gpu_3d_amd import scenes
         # the metric's FP32 vs FP64 pairs: the same 1M workload in FP32 + Kahan, and the 8M
@@ -433,7 +442,20 @@ def main():
                                       f"[2e4, 2e6] (4:2:1 blocks), 70% FP32+Kahan / 30% FP64, 100-1000 steps"),
                          "requests": len(reqs), "p50_time_to_result_s": sres["p50_s"],
                          "p99_time_to_result_s": sres["p99_s"], "particle_steps_per_s": sres["particle_steps_per_s"],
-                         "makespan_s": sres["makespan_s"]}
+                         "makespan_s": sres["makespan_s"], "mode": "batched, graph-replayed steps"}
+        if args.batched_instances > 0:
+            from iwf_mfree_gpu_3d_amd import serve
+            reqs = serve.sweep_requests(count=args.batched_instances, batched=True, seed=args.seed, n_min=5e3,
+                                        n_max=2e5, steps=(300, 300), fp32_share=1.0)
+            bres = serve.serve(reqs, hbm_budget=args.hbm_budget_gb * 1e9, streams=args.streams)
+            log(f"batched: {len(reqs)} instances, {bres['particle_steps_per_s']:.4e} particle-steps/s, "
+                f"p99 {bres['p99_s']:.2f} s over {bres['makespan_s']:.1f} s")
+            batched_rec = {"workload": (f"batched: {len(reqs)} orthogonal-cut instances at t=0, N log-uniform in "
+                                        "[5e3, 2e5], FP32+Kahan, 300 steps each; one launch per kernel over all "
+                                        "instances, steps replayed from a captured graph"),
+                           "particles": int(sum(r.lattice()[0] * r.lattice()[1] * r.lattice()[2] for r in reqs)),
+                           "particle_steps_per_s": bres["particle_steps_per_s"], "p50_time_to_result_s": bres["p50_s"],
+                           "p99_time_to_result_s": bres["p99_s"], "makespan_s": bres["makespan_s"]}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args)
@@ -462,6 +484,7 @@ def main():
         "whole_step": step,
         "precision_pairs": precision_pairs,
         "sweep": sweep_rec,
+        "batched": batched_rec,
         "kernels": res["kernels"],
         "cpu_baseline": cpu,
     }

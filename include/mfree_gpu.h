@@ -118,6 +118,7 @@ typedef struct mfg_tool mfg_tool;
 typedef struct mfg_tool_forces mfg_tool_forces;
 typedef struct mfg_leap_frog mfg_leap_frog;
 typedef struct mfg_scene mfg_scene;
+typedef struct mfg_batch mfg_batch;
 
 /* blanking box as blanking::get_bb / get_max_vel_squared return it after blanking::update
  * (src/blanking.h:57-63, src/blanking.cpp:43-67) */
@@ -182,6 +183,15 @@ int mfg_particle_gpu_counts(mfg_particles *p, unsigned *N, unsigned *N_init);
 int mfg_particle_gpu_precision(const mfg_particles *p);
 int mfg_particle_gpu_download(mfg_particles *p, int field, double *host);
 int mfg_particle_gpu_upload(mfg_particles *p, int field, const double *host);
+/* No counterpart in the reference (its particle_gpu lives as long as the process): a snapshot of
+ * a particle set's persistent state (every per-particle field, flags and persistent ids, N) as of
+ * its last queued step, into `bytes` of host memory, and its restore into a set of the same
+ * N_init and precision (e.g. after the instance was evicted from the GPU and set up again).  The
+ * state a step rebuilds (records, lists, cell tables) is not part of it.  Together with the
+ * tools' motion state and the caller's clock, the next step continues bit-identically. */
+int mfg_particle_gpu_snapshot_bytes(const mfg_particles *p, unsigned long long *bytes);
+int mfg_particle_gpu_snapshot(mfg_particles *p, void *host);
+int mfg_particle_gpu_restore(mfg_particles *p, const void *host);
 int mfg_field_width(int field);
 /* work of the last neighbour pass: candidate pairs visited and interacting pairs kept
  * (synchronises); no counterpart in the reference, used for the roofline accounting */
@@ -238,6 +248,10 @@ int mfg_tool_set_thermal(mfg_tool *t, int thermal);              /* :719 */
 int mfg_tool_set_physics(mfg_tool *t, const mfg_phys_constants *phys); /* :755 */
 int mfg_tool_get_vel(const mfg_tool *t, double vel[3]);          /* :703 */
 int mfg_tool_get_bbox(const mfg_tool *t, double bbmin[3], double bbmax[3]); /* :727 */
+/* motion state of a tool for snapshots (no counterpart in the reference): the time its triangles
+ * were last moved to, then its moving box (bbmin[3], bbmax[3]) */
+int mfg_tool_get_motion(const mfg_tool *t, double motion7[7]);
+int mfg_tool_set_motion(mfg_tool *t, const double motion7[7]);
 int mfg_tool_get_mu(const mfg_tool *t, double *mu);              /* :715 */
 int mfg_tool_is_alive(const mfg_tool *t, double t_current);      /* :737, returns 0/1 */
 /* triangle grid: dims[0..2] = nx, ny, nz, dims[3] = boxed triangles, dims[4] = triangles;
@@ -283,6 +297,36 @@ int mfg_leap_frog_step_noalloc(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g,
                                int record_forces, mfg_tool *const *tools, int n_tools,
                                const mfg_blanking_state *blanking, mfg_tool_forces *forces,
                                double dt, double t_current);
+/* ---- batched steps ---------------------------------------------------------------------------
+ * No counterpart in the reference, which steps one instance per process (leap_frog::step,
+ * src/leap_frog.cu:121-195, driven by src/mfree_gpu_3d.cpp:155-205).  A batch advances many
+ * independent instances together: every kernel of the step is launched once over all of them
+ * (each instance with its own cell grid, constants, tools and clock in a device descriptor), and
+ * a step can be captured into a HIP graph and replayed.  Each instance computes exactly what
+ * mfg_leap_frog_step computes for it alone (same kernel bodies; the pair pass in the sorted
+ * visiting order, which leaves results unchanged).
+ *
+ * mfg_batch_add takes the instance's objects, blanking state (NULL: none), force record (NULL:
+ * none), time step and clock (global_time_current before its next step); the library's current
+ * constants (the *_setup_* calls and mfg_set_options) become the instance's constants.  The
+ * instances of a batch share the precision mode and the heat model; each needs a locked grid
+ * that does not move, and a blanking box that does not move.  While an instance is in a batch,
+ * only mfg_batch_step advances it; the library keeps its objects (state buffers, tool motion) and
+ * clock in step on the host, so after mfg_batch_remove, or between batch steps, it can be read
+ * (mfg_particle_gpu_download) or stepped alone.  mfg_batch_step queues n_steps steps of every
+ * instance on the current stream (record_forces: every instance's force record holds the last
+ * step's forces); use_graph: the step is captured once into a graph (again after the batch
+ * changes) and replayed.  slot: the instance's index, which later removals shift down. */
+int mfg_batch_create(mfg_batch **out);
+int mfg_batch_destroy(mfg_batch *b);
+int mfg_batch_add(mfg_batch *b, mfg_particles *p, mfg_grid *g, mfg_tool *const *tools,
+                  int n_tools, const mfg_blanking_state *blanking, mfg_tool_forces *forces,
+                  double dt, double t_current, int *slot);
+int mfg_batch_remove(mfg_batch *b, int slot);
+int mfg_batch_size(const mfg_batch *b);
+int mfg_batch_get_time(const mfg_batch *b, int slot, double *t);
+int mfg_batch_step(mfg_batch *b, int n_steps, int record_forces, int use_graph);
+
 /* debug_check_valid_full (src/actions_gpu.cu:739-745): MFG_ERR_NONFINITE if an unblanked
  * particle holds a non-finite value */
 int mfg_debug_check_valid(mfg_particles *p);

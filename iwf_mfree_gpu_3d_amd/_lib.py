@@ -129,6 +129,9 @@ SIGNATURES = {
     "mfg_particle_gpu_precision": (_i, [_vp]),
     "mfg_particle_gpu_download": (_i, [_vp, _i, _dp]),
     "mfg_particle_gpu_upload": (_i, [_vp, _i, _dp]),
+    "mfg_particle_gpu_snapshot_bytes": (_i, [_vp, C.POINTER(C.c_ulonglong)]),
+    "mfg_particle_gpu_snapshot": (_i, [_vp, _vp]),
+    "mfg_particle_gpu_restore": (_i, [_vp, _vp]),
     "mfg_field_width": (_i, [_i]),
     "mfg_particle_gpu_pair_stats": (_i, [_vp, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]),
     "mfg_grid_create_locked": (_i, [_i, _i, _dp, _dp, _d, C.POINTER(_vp)]),
@@ -158,6 +161,8 @@ SIGNATURES = {
     "mfg_tool_set_physics": (_i, [_vp, C.POINTER(phys_constants)]),
     "mfg_tool_get_vel": (_i, [_vp, _dp]),
     "mfg_tool_get_bbox": (_i, [_vp, _dp, _dp]),
+    "mfg_tool_get_motion": (_i, [_vp, _dp]),
+    "mfg_tool_set_motion": (_i, [_vp, _dp]),
     "mfg_tool_get_mu": (_i, [_vp, _dp]),
     "mfg_tool_is_alive": (_i, [_vp, _d]),
     "mfg_tool_get_grid": (_i, [_vp, _lp, _dp]),
@@ -174,6 +179,13 @@ SIGNATURES = {
                                 _d, _d]),
     "mfg_leap_frog_step_noalloc": (_i, [_vp, _vp, _vp, _i, C.POINTER(_vp), _i, C.POINTER(blanking_state),
                                         _vp, _d, _d]),
+    "mfg_batch_create": (_i, [C.POINTER(_vp)]),
+    "mfg_batch_destroy": (_i, [_vp]),
+    "mfg_batch_add": (_i, [_vp, _vp, _vp, C.POINTER(_vp), _i, C.POINTER(blanking_state), _vp, _d, _d, _ip]),
+    "mfg_batch_remove": (_i, [_vp, _i]),
+    "mfg_batch_size": (_i, [_vp]),
+    "mfg_batch_get_time": (_i, [_vp, _i, _dp]),
+    "mfg_batch_step": (_i, [_vp, _i, _i, _i]),
     "mfg_debug_check_valid": (_i, [_vp]),
     "mfg_profile_enable": (_i, [_i]),
     "mfg_profile_reset": (_i, []),

@@ -213,6 +213,18 @@ class particle_gpu:
         v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
         check(_L().mfg_particle_gpu_upload(self._h, field, _dptr(v)))
 
+    def snapshot(self) -> np.ndarray:
+        """The persistent state as of the last queued step (bytes; see mfg_particle_gpu_snapshot)."""
+        n = C.c_ulonglong(0)
+        check(_L().mfg_particle_gpu_snapshot_bytes(self._h, C.byref(n)))
+        buf = np.empty(n.value, dtype=np.uint8)
+        check(_L().mfg_particle_gpu_snapshot(self._h, buf.ctypes.data_as(C.c_void_p)))
+        return buf
+
+    def restore(self, snap: np.ndarray) -> None:
+        buf = np.ascontiguousarray(snap, dtype=np.uint8)
+        check(_L().mfg_particle_gpu_restore(self._h, buf.ctypes.data_as(C.c_void_p)))
+
 
 # ---- grid ----------------------------------------------------------------------------------
 class grid_base:
@@ -361,6 +373,16 @@ class tool_3d_gpu:
         check(_L().mfg_tool_get_bbox(self._h, _dptr(a), _dptr(b)))
         return a, b
 
+    def get_motion(self) -> np.ndarray:
+        """(time the triangles were last moved to, bbmin[3], bbmax[3]) for snapshots."""
+        m = np.zeros(7)
+        check(_L().mfg_tool_get_motion(self._h, _dptr(m)))
+        return m
+
+    def set_motion(self, m) -> None:
+        v = np.ascontiguousarray(m, dtype=np.float64).reshape(7)
+        check(_L().mfg_tool_set_motion(self._h, _dptr(v)))
+
     def mu(self):
         m = C.c_double(0.)
         check(_L().mfg_tool_get_mu(self._h, C.byref(m)))
@@ -498,6 +520,66 @@ class leap_frog:
         fn = _L().mfg_leap_frog_step_noalloc if allocation_free else _L().mfg_leap_frog_step
         check(fn(self._h, particles._h, g._h, int(bool(record_forces and forces)), arr, len(tools),
                  C.byref(bstate) if bstate is not None else None, forces, global_time_dt, global_time_current))
+
+
+# ---- batched steps (no counterpart in the reference) -------------------------------------------
+class batch:
+    """Many independent instances stepped together: every kernel of the step runs once over all
+    of them, each instance with its own grid, constants, tools and clock (mfg_batch_*).  An
+    instance takes the library's constants as they are when it is added, so set them up for the
+    instance (as a reference set-up does) right before `add`.  Results per instance are those of
+    leap_frog.step on it alone."""
+
+    def __init__(self):
+        _lib.init()
+        handle = C.c_void_p()
+        check(_L().mfg_batch_create(C.byref(handle)))
+        self._h = handle
+        self._members = []  # the objects each instance uses stay alive while it is in the batch
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _L is not None:
+            _L().mfg_batch_destroy(h)
+            self._h = None
+
+    def __len__(self) -> int:
+        return len(self._members)
+
+    def add(self, particles: particle_gpu, g: grid_base, tools=(), bl: "blanking | None" = None,
+            forces: "tool_forces | None" = None, dt: float | None = None, t_current: float | None = None) -> int:
+        """Adds an instance (defaults: the process globals' time step and clock); returns its slot."""
+        tools = list(tools)
+        arr = (C.c_void_p * max(len(tools), 1))(*[t._h for t in tools])
+        bstate = None
+        if bl is not None:
+            if not bl.is_static():
+                raise ValueError("a batched instance needs a blanking box that does not move")
+            bl.update()
+            bstate = bl.state()
+        slot = C.c_int(-1)
+        check(_L().mfg_batch_add(self._h, particles._h, g._h, arr, len(tools),
+                                 C.byref(bstate) if bstate is not None else None,
+                                 forces._h if forces is not None else None,
+                                 global_time_dt if dt is None else float(dt),
+                                 global_time_current if t_current is None else float(t_current), C.byref(slot)))
+        self._members.append((particles, g, tools, bl, forces))
+        return slot.value
+
+    def remove(self, slot: int) -> None:
+        check(_L().mfg_batch_remove(self._h, int(slot)))
+        del self._members[slot]
+
+    def time(self, slot: int) -> float:
+        """The instance's clock (global_time_current before its next step)."""
+        t = C.c_double(0.0)
+        check(_L().mfg_batch_get_time(self._h, int(slot), C.byref(t)))
+        return t.value
+
+    def step(self, n_steps: int = 1, record_forces: bool = False, use_graph: bool = False) -> None:
+        """Queues n_steps steps of every instance on the current stream; use_graph: one captured
+        step replayed n_steps times."""
+        check(_L().mfg_batch_step(self._h, int(n_steps), int(bool(record_forces)), int(bool(use_graph))))
 
 
 # ---- profiling ---------------------------------------------------------------------------------

@@ -923,6 +923,66 @@ int mfg_particle_gpu_counts(mfg_particles *p, unsigned *N, unsigned *N_init) {
   return rc;
 }
 
+// snapshot layout: header, then the current state buffer as the device holds it (the double
+// arena, STATE_DOUBLES arrays of padded(N_init), and the flag / id arena, 2 x padded(N_init) u32)
+struct SnapshotHeader {
+  unsigned long long magic;
+  unsigned N_init, N;
+  int precision, n_doubles;
+  unsigned long long padded_n;
+};
+constexpr unsigned long long SNAPSHOT_MAGIC = 0x6d66672d736e6170ull;  // "mfg-snap"
+
+int mfg_particle_gpu_snapshot_bytes(const mfg_particles *p, unsigned long long *bytes) {
+  if (!p || !bytes) return fail(MFG_ERR_ARG, "null argument");
+  const size_t n = padded(p->N_init);
+  *bytes = sizeof(SnapshotHeader) + sizeof(double) * STATE_DOUBLES * n + sizeof(uint32_t) * 2 * n;
+  return MFG_OK;
+}
+
+int mfg_particle_gpu_snapshot(mfg_particles *p, void *host) {
+  NEED_LIB();
+  if (!p || !host) return fail(MFG_ERR_ARG, "null argument");
+  int rc = check_counters(p);  // waits for the queued steps; N of the last one
+  if (rc != MFG_OK && rc != MFG_ERR_ALL_BLANKED) return rc;
+  const size_t n = padded(p->N_init);
+  SnapshotHeader h{SNAPSHOT_MAGIC, p->N_init, p->N_host, p->precision, STATE_DOUBLES, n};
+  unsigned char *dst = (unsigned char *)host;
+  std::memcpy(dst, &h, sizeof(h));
+  dst += sizeof(h);
+  HIP_TRY(hipMemcpy(dst, p->arena[p->cur], sizeof(double) * STATE_DOUBLES * n, hipMemcpyDeviceToHost));
+  dst += sizeof(double) * STATE_DOUBLES * n;
+  HIP_TRY(hipMemcpy(dst, p->uarena[p->cur], sizeof(uint32_t) * 2 * n, hipMemcpyDeviceToHost));
+  return MFG_OK;
+}
+
+int mfg_particle_gpu_restore(mfg_particles *p, const void *host) {
+  NEED_LIB();
+  if (!p || !host) return fail(MFG_ERR_ARG, "null argument");
+  SnapshotHeader h;
+  std::memcpy(&h, host, sizeof(h));
+  const size_t n = padded(p->N_init);
+  if (h.magic != SNAPSHOT_MAGIC || h.n_doubles != STATE_DOUBLES || h.padded_n != n)
+    return fail(MFG_ERR_ARG, "not a snapshot of this library's particle layout");
+  if (h.N_init != p->N_init || h.precision != p->precision || h.N > h.N_init)
+    return fail(MFG_ERR_ARG, "snapshot of %u particles (precision %d) into a set of %u (precision %d)",
+                h.N_init, h.precision, p->N_init, p->precision);
+  hipStream_t s = g_lib.stream;
+  HIP_TRY(hipStreamSynchronize(s));  // nothing queued may still use the buffers
+  const unsigned char *src = (const unsigned char *)host + sizeof(h);
+  HIP_TRY(hipMemcpy(p->arena[p->cur], src, sizeof(double) * STATE_DOUBLES * n, hipMemcpyHostToDevice));
+  src += sizeof(double) * STATE_DOUBLES * n;
+  HIP_TRY(hipMemcpy(p->uarena[p->cur], src, sizeof(uint32_t) * 2 * n, hipMemcpyHostToDevice));
+  // N as the snapshot's last step left it; errors of the set's earlier life are cleared
+  Counters c0;
+  std::memset(&c0, 0, sizeof(c0));
+  c0.N = h.N;
+  HIP_TRY(hipMemcpy(p->cnt, &c0, sizeof(c0), hipMemcpyHostToDevice));
+  p->N_host = h.N;
+  p->stat_n = 0;
+  return MFG_OK;
+}
+
 int mfg_particle_gpu_download(mfg_particles *p, int f, double *host) {
   NEED_LIB();
   if (!p || !host) return fail(MFG_ERR_ARG, "null argument");
@@ -1463,6 +1523,26 @@ int mfg_tool_get_bbox(const mfg_tool *t, double mn[3], double mx[3]) {
   }
   return MFG_OK;
 }
+int mfg_tool_get_motion(const mfg_tool *t, double motion7[7]) {
+  if (!t || !motion7) return fail(MFG_ERR_ARG, "null argument");
+  motion7[0] = t->mesh_tc;
+  for (int k = 0; k < 3; k++) {
+    motion7[1 + k] = t->bbmin[k];
+    motion7[4 + k] = t->bbmax[k];
+  }
+  return MFG_OK;
+}
+int mfg_tool_set_motion(mfg_tool *t, const double motion7[7]) {
+  NEED_LIB();
+  if (!t || !motion7) return fail(MFG_ERR_ARG, "null argument");
+  t->mesh_tc = motion7[0];
+  for (int k = 0; k < 3; k++) {
+    t->bbmin[k] = motion7[1 + k];
+    t->bbmax[k] = motion7[4 + k];
+  }
+  // the triangles follow at the next step (mfg_tool_update moves them to that step's time)
+  return MFG_OK;
+}
 int mfg_tool_get_mu(const mfg_tool *t, double *mu) {
   if (!t || !mu) return fail(MFG_ERR_ARG, "null argument");
   *mu = t->mu;
@@ -1801,6 +1881,389 @@ int mfg_leap_frog_step_noalloc(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g,
   w.base = g_scratch;
   w.bytes = MFG_STEP_SCRATCH_BYTES;
   return step_impl(lf, p, g, record, tools, n_tools, blanking, forces, dt, t_current, w);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// batched steps: many instances, every kernel of the step launched once over all of them
+// ---------------------------------------------------------------------------------------------
+struct mfg_batch {
+  struct Member {
+    mfg_particles *p;
+    mfg_grid *g;
+    std::vector<mfg_tool *> tools;
+    mfg_tool_forces *forces;
+    int blanking;
+    double bb[6], vmax2;
+    Consts c;
+    double t;    // the instance's clock; the device keeps the same sequence (t += dt per step)
+    int search;  // lane-parallel exhaustive contact search
+  };
+  std::vector<Member> members;
+  int precision = -1, pse = -1;
+  bool dirty = true;   // the device descriptors and maps must be rebuilt before the next step
+  int table_record = -1;
+  Inst *d_inst = nullptr;
+  size_t inst_cap = 0;
+  unsigned *d_maps = nullptr;
+  size_t maps_cap = 0;
+  BatchLaunch L{};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t graph_stream = nullptr;
+  int graph_record = -1;
+};
+
+namespace {
+
+constexpr int SCAN_TILE_CELLS = 256 * 16;  // cells per scan tile of the counting sort
+constexpr unsigned BATCH_SEARCH_BLOCKS = 64;
+
+unsigned round8(unsigned n) { return (n + 7u) / 8u * 8u; }
+
+void batch_drop_graph(mfg_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->graph_record = -1;
+}
+
+// descriptor of member m as its objects hold it now
+Inst batch_inst(const mfg_batch::Member &m, int record) {
+  Inst d;
+  std::memset(&d, 0, sizeof(d));
+  mfg_particles *p = m.p;
+  d.buf[0] = p->buf[0];
+  d.buf[1] = p->buf[1];
+  d.cur = p->cur;
+  d.n_init = p->N_init;
+  d.g = m.g->info();
+  d.gb = grid_buffers(m.g, p);
+  d.nparts = (int)((m.g->num_cell + SCAN_TILE_CELLS - 1) / SCAN_TILE_CELLS);
+  d.blanking = m.blanking;
+  for (int k = 0; k < 6; k++) d.bb[k] = m.bb[k];
+  d.vmax2 = m.vmax2;
+  d.c = m.c;
+  d.rec = p->rec;
+  d.nl4 = p->nl4;
+  d.pse2 = p->pse2;
+  d.cs4 = p->cs4;
+  d.nbr = p->nbr;
+  d.count = p->count;
+  d.poison = p->poison;
+  d.nl_stride = p->nl_stride;
+  d.contact_blocks = p->contact_blocks;
+  d.sums = p->sums;
+  d.sum_stride = p->sum_stride;
+  d.cnt = p->cnt;
+  d.partials = p->partials;
+  d.forces = m.forces ? m.forces->d : nullptr;
+  d.n_forces = m.forces ? (int)m.forces->n : 0;
+  d.search = m.search;
+  if (m.search) {
+    d.cs.list = p->cs_list;
+    d.cs.list_n = p->cs_n;
+    d.cs.cap = p->cs_cap;
+    d.cs.slot = p->cs_slot;
+    d.cs.res = p->cs_res;
+  }
+  const int n_tools = (int)m.tools.size();
+  d.ts.n = n_tools;
+  d.ts.record = record ? 1 : 0;
+  if (n_tools > 0)
+    for (int a = 0; a < 3; a++) d.ts.fric_vel[a] = m.tools[0]->vel[a];
+  for (int k = 0; k < n_tools; k++) {
+    const mfg_tool *t = m.tools[k];
+    ToolView &v = d.ts.t[k];
+    v.algo = t->algo;
+    v.tris = t->algo == MFG_CONTACT_HASHED ? t->boxed : t->tris;
+    v.n_tris = t->algo == MFG_CONTACT_HASHED ? t->n_boxed : t->n_tris;
+    v.cells = t->cells;
+    v.cand_off = t->algo == MFG_CONTACT_EXHAUSTIVE ? t->cand_off : nullptr;
+    v.cand_idx = t->cand_idx;
+    if (m.search && t->algo == MFG_CONTACT_EXHAUSTIVE) {
+      v.slot = p->cs_slot + (size_t)k * p->N_init;
+      v.res = p->cs_res;
+    }
+    for (int a = 0; a < 3; a++) v.corg[a] = t->gbbmin[a] + t->vel[a] * t->mesh_tc;
+    v.nx = t->gnx; v.ny = t->gny; v.nz = t->gnz;
+    v.active = 0;  // set by the tool kernel from the instance's clock every step
+    for (int a = 0; a < 3; a++) {
+      v.bbmin[a] = t->bbmin[a];
+      v.bbmax[a] = t->bbmax[a];
+      v.vel_const[a] = t->vel_const[a];
+    }
+    v.dx = t->gdx;
+    v.mu = t->mu;
+    v.alpha = t->alpha;
+    v.mass = t->mass;
+    ToolMotion &tm = d.tm[k];
+    tm.tris = t->tris;
+    tm.p_init = t->p_init_all;
+    tm.boxed = t->boxed;
+    tm.p_init_boxed = t->p_init_boxed;
+    tm.n_tris = t->n_tris;
+    tm.n_boxed = t->n_boxed;
+    tm.algo = t->algo;
+    for (int a = 0; a < 3; a++) {
+      tm.vel[a] = t->vel[a];
+      tm.gbbmin[a] = t->gbbmin[a];
+    }
+    tm.mesh_tc = t->mesh_tc;
+    tm.birth = t->birth;
+    tm.death = t->death;
+  }
+  d.t = m.t;
+  return d;
+}
+
+// descriptors and workgroup maps of every member on the device
+int batch_build(mfg_batch *b, int record) {
+  const int n = (int)b->members.size();
+  batch_drop_graph(b);
+  if (n == 0) {
+    b->dirty = false;
+    return MFG_OK;
+  }
+  if ((size_t)n > b->inst_cap) {
+    if (b->d_inst) (void)hipFree(b->d_inst);
+    b->d_inst = nullptr;
+    b->inst_cap = 0;
+    HIP_TRY(hipMalloc(&b->d_inst, sizeof(Inst) * (size_t)n));
+    b->inst_cap = n;
+  }
+  constexpr int N_MAPS = 9;
+  const size_t map_len = (size_t)n + 1;
+  if (N_MAPS * map_len > b->maps_cap) {
+    if (b->d_maps) (void)hipFree(b->d_maps);
+    b->d_maps = nullptr;
+    b->maps_cap = 0;
+    HIP_TRY(hipMalloc(&b->d_maps, sizeof(unsigned) * N_MAPS * map_len));
+    b->maps_cap = N_MAPS * map_len;
+  }
+  std::vector<Inst> host(n);
+  std::vector<unsigned> maps(N_MAPS * map_len, 0u);
+  BatchGrid *grids[N_MAPS] = {&b->L.cells1, &b->L.cells, &b->L.tiles, &b->L.one, &b->L.parts,
+                              &b->L.parts8, &b->L.nbrs8, &b->L.tools, &b->L.search};
+  for (int i = 0; i < n; i++) {
+    const mfg_batch::Member &m = b->members[i];
+    host[i] = batch_inst(m, record);
+    const GridInfo gi = m.g->info();
+    const unsigned nblk = (unsigned)m.p->nblk;
+    const unsigned blocks[N_MAPS] = {
+        (unsigned)((gi.num_cell + 1 + 255) / 256),
+        (unsigned)((gi.num_cell + 255) / 256),
+        (unsigned)host[i].nparts,
+        1u,
+        nblk,
+        round8(nblk),
+        round8((unsigned)neighbour_blocks(gi)),
+        (unsigned)std::max<size_t>(1, m.tools.size()),
+        m.search ? std::min(BATCH_SEARCH_BLOCKS, std::max(1u, nblk)) : 0u};
+    for (int k = 0; k < N_MAPS; k++) maps[k * map_len + i + 1] = maps[k * map_len + i] + blocks[k];
+  }
+  for (int k = 0; k < N_MAPS; k++) {
+    const unsigned total = maps[k * map_len + n];
+    if (total > 0x7fffffffu) return fail(MFG_ERR_ARG, "batched launch of %u workgroups", total);
+    grids[k]->m.off = b->d_maps + k * map_len;
+    grids[k]->m.n = n;
+    grids[k]->blocks = total;
+  }
+  hipStream_t s = g_lib.stream;
+  HIP_TRY(hipMemcpyAsync(b->d_inst, host.data(), sizeof(Inst) * n, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(b->d_maps, maps.data(), sizeof(unsigned) * maps.size(),
+                         hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));  // the host arrays go out of scope
+  b->L.inst = b->d_inst;
+  b->L.n = n;
+  b->L.precision = b->precision;
+  b->L.pse = b->pse;
+  b->table_record = record;
+  b->dirty = false;
+  return MFG_OK;
+}
+
+void batch_launch_step(mfg_batch *b, int record) {
+  hipStream_t s = g_lib.stream;
+  {
+    ProfScope ps("batch_sort");
+    launch_batch_sort(s, b->L);
+  }
+  {
+    ProfScope ps("batch_permute_predict");
+    launch_batch_permute(s, b->L);
+  }
+  {
+    ProfScope ps("batch_pair");
+    launch_batch_pair(s, b->L);
+  }
+  {
+    ProfScope ps("batch_contact");
+    launch_batch_contact(s, b->L, record);
+  }
+}
+
+// what the device does to an instance's host-visible state in one step, mirrored on the host
+// with the same arithmetic (tool motion as mfg_tool_update, clock, state buffer)
+void batch_mirror_step(mfg_batch::Member &m) {
+  const double dt = m.c.dt;
+  for (mfg_tool *t : m.tools) {
+    if (t->n_boxed == 0) continue;
+    t->mesh_tc = m.t;
+    for (int k = 0; k < 3; k++) {
+      t->bbmin[k] += dt * t->vel[k];
+      t->bbmax[k] += dt * t->vel[k];
+    }
+  }
+  m.t = m.t + dt;
+  m.p->cur ^= 1;
+  m.p->stat_n = 0;
+  m.g->scanned = true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mfg_batch_create(mfg_batch **out) {
+  NEED_LIB();
+  if (!out) return fail(MFG_ERR_ARG, "null output");
+  *out = new mfg_batch();
+  return MFG_OK;
+}
+
+int mfg_batch_destroy(mfg_batch *b) {
+  if (!b) return MFG_OK;
+  batch_drop_graph(b);
+  if (b->d_inst) (void)hipFree(b->d_inst);
+  if (b->d_maps) (void)hipFree(b->d_maps);
+  delete b;
+  return MFG_OK;
+}
+
+int mfg_batch_add(mfg_batch *b, mfg_particles *p, mfg_grid *g, mfg_tool *const *tools,
+                  int n_tools, const mfg_blanking_state *blanking, mfg_tool_forces *forces,
+                  double dt, double t_current, int *slot) {
+  NEED_LIB();
+  if (!b || !p || !g) return fail(MFG_ERR_ARG, "null argument");
+  if (n_tools < 0 || n_tools > MAX_TOOLS || (n_tools > 0 && !tools))
+    return fail(MFG_ERR_ARG, "%d tools (max %d)", n_tools, MAX_TOOLS);
+  for (int k = 0; k < n_tools; k++)
+    if (!tools[k]) return fail(MFG_ERR_ARG, "tool %d is null", k);
+  if (forces && forces->n < (unsigned)n_tools)
+    return fail(MFG_ERR_ARG, "the force record holds %u tools, the instance has %d", forces->n,
+                n_tools);
+  if (!g->locked || g->vel_bbox[0] != 0. || g->vel_bbox[1] != 0. || g->vel_bbox[2] != 0.)
+    return fail(MFG_ERR_ARG, "a batched instance needs a locked grid that does not move");
+  if (g->num_cell <= 0) return fail(MFG_ERR_ARG, "the grid has no cells");
+  Consts c = make_consts(dt, p->precision);
+  const int pse = (c.heat == MFG_HEAT_PSE && c.pse_on) ? 1 : 0;
+  if (b->precision >= 0 && (b->precision != p->precision || b->pse != pse))
+    return fail(MFG_ERR_ARG,
+                "instances of a batch share the precision mode and the heat model (batch: "
+                "precision %d, PSE %d; instance: %d, %d)",
+                b->precision, b->pse, p->precision, pse);
+  for (const mfg_batch::Member &m : b->members) {
+    if (m.p == p || m.g == g)
+      return fail(MFG_ERR_ARG, "the particle set or grid is already in the batch");
+    for (mfg_tool *t : m.tools)
+      for (int k = 0; k < n_tools; k++)
+        if (t == tools[k]) return fail(MFG_ERR_ARG, "tool %d already moves with another instance", k);
+  }
+  mfg_batch::Member m;
+  m.p = p;
+  m.g = g;
+  m.tools.assign(tools, tools + n_tools);
+  m.forces = forces;
+  m.blanking = blanking ? 1 : 0;
+  for (int k = 0; k < 3; k++) {
+    m.bb[k] = blanking ? blanking->bbmin[k] : 0.;
+    m.bb[3 + k] = blanking ? blanking->bbmax[k] : 0.;
+  }
+  m.vmax2 = blanking ? blanking->max_vel_squared : FLT_MAX;
+  m.c = c;
+  m.t = t_current;
+  int n_exh = 0;
+  for (int k = 0; k < n_tools; k++)
+    n_exh += (tools[k]->algo == MFG_CONTACT_EXHAUSTIVE && tools[k]->n_boxed > 0);
+  m.search = (n_exh > 0 && g_lib.opt_contact_search) ? 1 : 0;
+  if (m.search) {
+    ContactSearch cs{};
+    int rc = contact_search_buffers(p, n_tools, cs);
+    if (rc != MFG_OK) return rc;
+  }
+  b->precision = p->precision;
+  b->pse = pse;
+  b->members.push_back(m);
+  b->dirty = true;
+  if (slot) *slot = (int)b->members.size() - 1;
+  return MFG_OK;
+}
+
+int mfg_batch_remove(mfg_batch *b, int slot) {
+  NEED_LIB();
+  if (!b || slot < 0 || slot >= (int)b->members.size())
+    return fail(MFG_ERR_ARG, "no instance %d in the batch", slot);
+  HIP_TRY(hipStreamSynchronize(g_lib.stream));  // its last steps may still be queued
+  b->members.erase(b->members.begin() + slot);
+  if (b->members.empty()) {
+    b->precision = -1;
+    b->pse = -1;
+  }
+  b->dirty = true;
+  return MFG_OK;
+}
+
+int mfg_batch_size(const mfg_batch *b) { return b ? (int)b->members.size() : MFG_ERR_ARG; }
+
+int mfg_batch_get_time(const mfg_batch *b, int slot, double *t) {
+  if (!b || !t || slot < 0 || slot >= (int)b->members.size())
+    return fail(MFG_ERR_ARG, "no instance %d in the batch", slot);
+  *t = b->members[slot].t;
+  return MFG_OK;
+}
+
+int mfg_batch_step(mfg_batch *b, int n_steps, int record_forces, int use_graph) {
+  NEED_LIB();
+  if (!b || n_steps < 0) return fail(MFG_ERR_ARG, "batch step: %d steps", n_steps);
+  if (b->members.empty() || n_steps == 0) return MFG_OK;
+  const int record = record_forces ? 1 : 0;
+  if (record)
+    for (const mfg_batch::Member &m : b->members)
+      if (!m.forces && !m.tools.empty())
+        return fail(MFG_ERR_ARG, "recording forces needs a force record for every instance");
+  if (b->dirty || b->table_record != record) {
+    int rc = batch_build(b, record);
+    if (rc != MFG_OK) return rc;
+  }
+  hipStream_t s = g_lib.stream;
+  if (use_graph) {
+    if (!b->exec || b->graph_record != record || b->graph_stream != s) {
+      batch_drop_graph(b);
+      ProfOff off(true);
+      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      batch_launch_step(b, record);
+      hipGraph_t gr = nullptr;
+      hipError_t e = hipStreamEndCapture(s, &gr);
+      if (e != hipSuccess || !gr) return fail(MFG_ERR_HIP, "capturing the batched step: %s", hipGetErrorString(e));
+      b->graph = gr;
+      HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+      b->graph_record = record;
+      b->graph_stream = s;
+    }
+    for (int k = 0; k < n_steps; k++) {
+      ProfScope ps("batch_step_graph");
+      HIP_TRY(hipGraphLaunch(b->exec, s));
+    }
+  } else {
+    for (int k = 0; k < n_steps; k++) batch_launch_step(b, record);
+  }
+  HIP_TRY(hipGetLastError());
+  for (int k = 0; k < n_steps; k++)
+    for (mfg_batch::Member &m : b->members) batch_mirror_step(m);
+  return MFG_OK;
 }
 
 

@@ -64,21 +64,27 @@ __device__ __forceinline__ bool rates_invalid(const State &st, unsigned i) {
          bad(st.T_t[i]);
 }
 
-__global__ void __launch_bounds__(BS) k_zero_cells(int *count, int *fill, long n) {
-  long c = (long)blockIdx.x * BS + threadIdx.x;
+// Each kernel below is a body run by workgroup `b` (blockIdx.x of a single-instance launch, the
+// workgroup inside its instance's range of a batched launch) and a __global__ wrapper.
+__device__ __forceinline__ void zero_cells(unsigned b, int *count, int *fill, long n) {
+  long c = (long)b * BS + threadIdx.x;
   if (c < n) {
     count[c] = 0;
     fill[c] = 0;
   }
   if (c == n) count[c] = 0;
 }
+__global__ void __launch_bounds__(BS) k_zero_cells(int *count, int *fill, long n) {
+  zero_cells(blockIdx.x, count, fill, n);
+}
 
-__global__ void __launch_bounds__(BS)
-    k_blank_hash_count(State st, unsigned n_init, GridInfo g, int blanking, double b0, double b1,
-                       double b2, double b3, double b4, double b5, double vmax2, int *key,
-                       int *count, int *blk, Counters *cnt) {
+__device__ __forceinline__ void blank_hash_count(unsigned b, const State &st, unsigned n_init,
+                                                 const GridInfo &g, int blanking, double b0,
+                                                 double b1, double b2, double b3, double b4,
+                                                 double b5, double vmax2, int *key, int *count,
+                                                 int *blk, Counters *cnt) {
   __shared__ int lds[BS / 64 + 1];
-  const unsigned i = blockIdx.x * BS + threadIdx.x;
+  const unsigned i = b * BS + threadIdx.x;
   int blank_key = 0;
   if (i < n_init) {
     uint32_t fl = st.flags[i];
@@ -115,20 +121,30 @@ __global__ void __launch_bounds__(BS)
   }
   int total;
   block_excl_scan<BS>(blank_key, lds, total);
-  if (threadIdx.x == 0) blk[blockIdx.x] = total;
+  if (threadIdx.x == 0) blk[b] = total;
+}
+__global__ void __launch_bounds__(BS)
+    k_blank_hash_count(State st, unsigned n_init, GridInfo g, int blanking, double b0, double b1,
+                       double b2, double b3, double b4, double b5, double vmax2, int *key,
+                       int *count, int *blk, Counters *cnt) {
+  blank_hash_count(blockIdx.x, st, n_init, g, blanking, b0, b1, b2, b3, b4, b5, vmax2, key, count,
+                   blk, cnt);
 }
 
 // tile reduction of the cell counts
-__global__ void __launch_bounds__(BS) k_scan_reduce(const int *count, long n, int *part) {
+__device__ __forceinline__ void scan_reduce(unsigned b, const int *count, long n, int *part) {
   __shared__ int lds[BS / 64 + 1];
-  long base = (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
+  long base = (long)b * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
   int s = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; k++)
     if (base + k < n) s += count[base + k];
   int total;
   block_excl_scan<BS>(s, lds, total);
-  if (threadIdx.x == 0) part[blockIdx.x] = total;
+  if (threadIdx.x == 0) part[b] = total;
+}
+__global__ void __launch_bounds__(BS) k_scan_reduce(const int *count, long n, int *part) {
+  scan_reduce(blockIdx.x, count, n, part);
 }
 
 // single-block exclusive scan of two small arrays (tile sums of the cells, block blank counts)
@@ -147,8 +163,8 @@ __device__ void block_scan_array(int *a, int n, int *lds, int &total) {
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(SB) k_scan_top(int *part, int nparts, int *blk, int nblk,
-                                                 int *count, long num_cell, Counters *cnt) {
+__device__ __forceinline__ void scan_top(int *part, int nparts, int *blk, int nblk, int *count,
+                                         long num_cell, Counters *cnt) {
   __shared__ int lds[SB / 64 + 1];
   int tot_valid, tot_blank;
   block_scan_array(part, nparts, lds, tot_valid);
@@ -161,10 +177,14 @@ __global__ void __launch_bounds__(SB) k_scan_top(int *part, int nparts, int *blk
     cnt->pairs = 0ull;
   }
 }
+__global__ void __launch_bounds__(SB) k_scan_top(int *part, int nparts, int *blk, int nblk,
+                                                 int *count, long num_cell, Counters *cnt) {
+  scan_top(part, nparts, blk, nblk, count, num_cell, cnt);
+}
 
-__global__ void __launch_bounds__(BS) k_scan_down(int *count, long n, const int *part) {
+__device__ __forceinline__ void scan_down(unsigned b, int *count, long n, const int *part) {
   __shared__ int lds[BS / 64 + 1];
-  long base = (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
+  long base = (long)b * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
   int v[SCAN_ITEMS];
   int s = 0;
 #pragma unroll
@@ -173,7 +193,7 @@ __global__ void __launch_bounds__(BS) k_scan_down(int *count, long n, const int 
     s += v[k];
   }
   int total;
-  int off = block_excl_scan<BS>(s, lds, total) + part[blockIdx.x];
+  int off = block_excl_scan<BS>(s, lds, total) + part[b];
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; k++)
     if (base + k < n) {
@@ -181,26 +201,34 @@ __global__ void __launch_bounds__(BS) k_scan_down(int *count, long n, const int 
       off += v[k];
     }
 }
+__global__ void __launch_bounds__(BS) k_scan_down(int *count, long n, const int *part) {
+  scan_down(blockIdx.x, count, n, part);
+}
 
-__global__ void __launch_bounds__(BS) k_scatter(unsigned n_init, const int *key, const int *off,
-                                                int *fill, const int *blk, const Counters *cnt,
-                                                int *order) {
+__device__ __forceinline__ void scatter(unsigned b, unsigned n_init, const int *key,
+                                        const int *off, int *fill, const int *blk,
+                                        const Counters *cnt, int *order) {
   __shared__ int lds[BS / 64 + 1];
-  const unsigned i = blockIdx.x * BS + threadIdx.x;
+  const unsigned i = b * BS + threadIdx.x;
   int k = (i < n_init) ? key[i] : 0;
   int is_blank = (i < n_init && k == KEY_BLANK) ? 1 : 0;
   int total;
   int rank = block_excl_scan<BS>(is_blank, lds, total);
   if (i >= n_init) return;
   int pos;
-  if (is_blank) pos = (int)cnt->N + blk[blockIdx.x] + rank;
+  if (is_blank) pos = (int)cnt->N + blk[b] + rank;
   else pos = off[k] + atomicAdd(&fill[k], 1);
   order[pos] = (int)i;
 }
+__global__ void __launch_bounds__(BS) k_scatter(unsigned n_init, const int *key, const int *off,
+                                                int *fill, const int *blk, const Counters *cnt,
+                                                int *order) {
+  scatter(blockIdx.x, n_init, key, off, fill, blk, cnt, order);
+}
 
 // restore previous-index order inside every cell (stable sort)
-__global__ void __launch_bounds__(BS) k_fixup(const int *off, long num_cell, int *order) {
-  long c = (long)blockIdx.x * BS + threadIdx.x;
+__device__ __forceinline__ void fixup(unsigned wg, const int *off, long num_cell, int *order) {
+  long c = (long)wg * BS + threadIdx.x;
   if (c >= num_cell) return;
   int s = off[c], e = off[c + 1];
   if (e - s < 2) return;
@@ -213,6 +241,9 @@ __global__ void __launch_bounds__(BS) k_fixup(const int *off, long num_cell, int
     }
     order[b + 1] = v;
   }
+}
+__global__ void __launch_bounds__(BS) k_fixup(const int *off, long num_cell, int *order) {
+  fixup(blockIdx.x, off, num_cell, order);
 }
 
 // what the CSPM sum reads of a neighbour, 32 bytes: FP64 (x, y, z, vol), the volume of a tool
@@ -277,11 +308,13 @@ __device__ __forceinline__ void store_rec<Rec32>(Rec32 *rec, unsigned i, const d
 // gather the persistent state by the sort permutation, then predict the half step and build the
 // pair records (EOS and artificial stress included)
 template <class RecT>
-__global__ void __launch_bounds__(BS)
-    k_permute_predict(State src, State dst, unsigned n_init, const int *order, const int *key,
-                      int *key_sorted, const Counters *cnt, Consts c, int predict, RecT *rec,
-                      double4 *nl4, double2 *pse2, double4 *cs4, double *p_dbg, double *R_dbg) {
-  const unsigned i = blockIdx.x * BS + threadIdx.x;
+__device__ __forceinline__ void permute_predict(unsigned b, const State &src, const State &dst,
+                                                unsigned n_init, const int *order,
+                                                const int *key, int *key_sorted,
+                                                const Counters *cnt, const Consts &c, int predict,
+                                                RecT *rec, double4 *nl4, double2 *pse2,
+                                                double4 *cs4, double *p_dbg, double *R_dbg) {
+  const unsigned i = b * BS + threadIdx.x;
   if (i >= n_init) return;
   const unsigned N = cnt->N;
   const int s = order[i];
@@ -370,6 +403,14 @@ __global__ void __launch_bounds__(BS)
 #pragma unroll
     for (int k = 0; k < 6; k++) R_dbg[6 * (size_t)i + k] = R[k];
   }
+}
+template <class RecT>
+__global__ void __launch_bounds__(BS)
+    k_permute_predict(State src, State dst, unsigned n_init, const int *order, const int *key,
+                      int *key_sorted, const Counters *cnt, Consts c, int predict, RecT *rec,
+                      double4 *nl4, double2 *pse2, double4 *cs4, double *p_dbg, double *R_dbg) {
+  permute_predict<RecT>(blockIdx.x, src, dst, n_init, order, key, key_sorted, cnt, c, predict, rec,
+                        nl4, pse2, cs4, p_dbg, R_dbg);
 }
 
 // Green (0xffffffff for empty cells) or Rothlin (start == end) tables from the offsets
@@ -485,6 +526,83 @@ void launch_bbox_reduce(hipStream_t s, const State &st, const Counters *cnt, uns
                         double *partials, int nb) {
   (void)n_init;
   hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(BS), 0, s, st, cnt, partials);
+}
+
+// ---------------------------------------------------------------------------------------------
+// batched launches: the same bodies, arguments from each instance's descriptor.  Before the
+// permutation an instance's state is buf[cur], after it buf[cur ^ 1].
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(BS) kb_zero_cells(const Inst *in, BatchMap m) {
+  unsigned b, nb;
+  const Inst &d = in[batch_locate(m, b, nb)];
+  zero_cells(b, d.gb.count, d.gb.fill, d.g.num_cell);
+}
+
+__global__ void __launch_bounds__(BS) kb_blank_hash_count(const Inst *in, BatchMap m) {
+  unsigned b, nb;
+  const Inst &d = in[batch_locate(m, b, nb)];
+  blank_hash_count(b, d.buf[d.cur], d.n_init, d.g, d.blanking, d.bb[0], d.bb[1], d.bb[2], d.bb[3],
+                   d.bb[4], d.bb[5], d.vmax2, d.gb.key, d.gb.count, d.gb.blk, d.cnt);
+}
+
+__global__ void __launch_bounds__(BS) kb_scan_reduce(const Inst *in, BatchMap m) {
+  unsigned b, nb;
+  const Inst &d = in[batch_locate(m, b, nb)];
+  scan_reduce(b, d.gb.count, d.g.num_cell, d.gb.blk2);
+}
+
+__global__ void __launch_bounds__(SB) kb_scan_top(const Inst *in, BatchMap m) {
+  unsigned b, nb;
+  const Inst &d = in[batch_locate(m, b, nb)];
+  scan_top(d.gb.blk2, d.nparts, d.gb.blk, d.gb.nblk, d.gb.count, d.g.num_cell, d.cnt);
+}
+
+__global__ void __launch_bounds__(BS) kb_scan_down(const Inst *in, BatchMap m) {
+  unsigned b, nb;
+  const Inst &d = in[batch_locate(m, b, nb)];
+  scan_down(b, d.gb.count, d.g.num_cell, d.gb.blk2);
+}
+
+__global__ void __launch_bounds__(BS) kb_scatter(const Inst *in, BatchMap m) {
+  unsigned b, nb;
+  const Inst &d = in[batch_locate(m, b, nb)];
+  scatter(b, d.n_init, d.gb.key, d.gb.count, d.gb.fill, d.gb.blk, d.cnt, d.gb.order);
+}
+
+__global__ void __launch_bounds__(BS) kb_fixup(const Inst *in, BatchMap m) {
+  unsigned b, nb;
+  const Inst &d = in[batch_locate(m, b, nb)];
+  fixup(b, d.gb.count, d.g.num_cell, d.gb.order);
+}
+
+template <class RecT>
+__global__ void __launch_bounds__(BS) kb_permute_predict(const Inst *in, BatchMap m) {
+  unsigned b, nb;
+  const Inst &d = in[batch_locate(m, b, nb)];
+  permute_predict<RecT>(b, d.buf[d.cur], d.buf[d.cur ^ 1], d.n_init, d.gb.order, d.gb.key,
+                        d.gb.key_sorted, d.cnt, d.c, 1, (RecT *)d.rec, d.nl4, d.pse2, d.cs4,
+                        nullptr, nullptr);
+}
+
+// hash + blanking, scan, scatter and fix-up of every instance (the sort permutation and cell
+// offsets), as sort_keys does for one
+void launch_batch_sort(hipStream_t s, const BatchLaunch &L) {
+  hipLaunchKernelGGL(kb_zero_cells, dim3(L.cells1.blocks), dim3(BS), 0, s, L.inst, L.cells1.m);
+  hipLaunchKernelGGL(kb_blank_hash_count, dim3(L.parts.blocks), dim3(BS), 0, s, L.inst, L.parts.m);
+  hipLaunchKernelGGL(kb_scan_reduce, dim3(L.tiles.blocks), dim3(BS), 0, s, L.inst, L.tiles.m);
+  hipLaunchKernelGGL(kb_scan_top, dim3(L.one.blocks), dim3(SB), 0, s, L.inst, L.one.m);
+  hipLaunchKernelGGL(kb_scan_down, dim3(L.tiles.blocks), dim3(BS), 0, s, L.inst, L.tiles.m);
+  hipLaunchKernelGGL(kb_scatter, dim3(L.parts.blocks), dim3(BS), 0, s, L.inst, L.parts.m);
+  hipLaunchKernelGGL(kb_fixup, dim3(L.cells.blocks), dim3(BS), 0, s, L.inst, L.cells.m);
+}
+
+void launch_batch_permute(hipStream_t s, const BatchLaunch &L) {
+  if (L.precision == MFG_FP64)
+    hipLaunchKernelGGL(kb_permute_predict<Rec64>, dim3(L.parts.blocks), dim3(BS), 0, s, L.inst,
+                       L.parts.m);
+  else
+    hipLaunchKernelGGL(kb_permute_predict<Rec32>, dim3(L.parts.blocks), dim3(BS), 0, s, L.inst,
+                       L.parts.m);
 }
 
 }  // namespace mfg

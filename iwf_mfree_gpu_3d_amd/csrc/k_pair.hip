@@ -361,14 +361,14 @@ __device__ __forceinline__ int range_of(const int *pref, int nr, int q) {
 #ifndef NB_WAVES_OTHER
 #define NB_WAVES_OTHER 8
 #endif
+// body of workgroup b of nb (the single-instance kernel and the batched one below)
 template <int PSE_MODE>
-__global__ void __launch_bounds__(NB_BLOCK)
-    __attribute__((amdgpu_waves_per_eu(PSE_MODE == 1 ? NB_WAVES_FP64_PSE : NB_WAVES_OTHER)))
-    k_cell_neighbours(GridInfo g, const int *__restrict__ off, const double4 *__restrict__ nl4,
-                      const double2 *__restrict__ pse2, const double *__restrict__ h,
-                      const uint32_t *__restrict__ flags, Consts c, int *__restrict__ nbr, int cap,
-                      int *__restrict__ count, double *__restrict__ T_t, Counters *cnt,
-                      unsigned long long *__restrict__ stat_part, int *__restrict__ poison) {
+__device__ __forceinline__ void cell_neighbours(
+    unsigned b, unsigned nb, const GridInfo &g, const int *__restrict__ off,
+    const double4 *__restrict__ nl4, const double2 *__restrict__ pse2,
+    const double *__restrict__ h, const uint32_t *__restrict__ flags, const Consts &c,
+    int *__restrict__ nbr, int cap, int *__restrict__ count, double *__restrict__ T_t,
+    Counters *cnt, unsigned long long *__restrict__ stat_part, int *__restrict__ poison) {
   __shared__ int s_cand[NB_WAVES][NB_LCAP];
   __shared__ int s_pref[NB_WAVES][32];
   __shared__ int s_rst[NB_WAVES][32];
@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(NB_BLOCK)
     if (threadIdx.x < 64) s_exp[threadIdx.x] = EXP2_J64[threadIdx.x];
     __syncthreads();
   }
-  const long cell = (long)xcd_block() * NB_WAVES + w;
+  const long cell = (long)xcd_map(b, nb) * NB_WAVES + w;
   unsigned long long n_cand = 0, n_pair = 0;
   const int s = cell < g.num_cell ? off[cell] : 0;
   const int e = cell < g.num_cell ? off[cell + 1] : 0;
@@ -439,9 +439,20 @@ __global__ void __launch_bounds__(NB_BLOCK)
       a += s_stat[k][0];
       bb += s_stat[k][1];
     }
-    stat_part[2 * blockIdx.x] = a;
-    stat_part[2 * blockIdx.x + 1] = bb;
+    stat_part[2 * b] = a;
+    stat_part[2 * b + 1] = bb;
   }
+}
+template <int PSE_MODE>
+__global__ void __launch_bounds__(NB_BLOCK)
+    __attribute__((amdgpu_waves_per_eu(PSE_MODE == 1 ? NB_WAVES_FP64_PSE : NB_WAVES_OTHER)))
+    k_cell_neighbours(GridInfo g, const int *__restrict__ off, const double4 *__restrict__ nl4,
+                      const double2 *__restrict__ pse2, const double *__restrict__ h,
+                      const uint32_t *__restrict__ flags, Consts c, int *__restrict__ nbr, int cap,
+                      int *__restrict__ count, double *__restrict__ T_t, Counters *cnt,
+                      unsigned long long *__restrict__ stat_part, int *__restrict__ poison) {
+  cell_neighbours<PSE_MODE>(blockIdx.x, gridDim.x, g, off, nl4, pse2, h, flags, c, nbr, cap, count,
+                            T_t, cnt, stat_part, poison);
 }
 
 // Runs body(load(row[k])) for k = 0 .. n-1 in order.  The neighbour indices are read four at a
@@ -699,10 +710,11 @@ __device__ __forceinline__ void store_sums(double *__restrict__ sums, size_t str
 
 // Jaumann rate, continuity, momentum, advection, invalidation, leap-frog correction and the
 // Johnson-Cook radial return of every particle the pair pass summed (not blanked, in a cell)
-__global__ void __launch_bounds__(BSP)
-    k_particle_update(unsigned n_init, State st, const double *__restrict__ sums, size_t stride,
-                      const int *__restrict__ poison, Consts c, const Counters *cnt) {
-  const unsigned i = blockIdx.x * BSP + threadIdx.x;
+__device__ __forceinline__ void update_body(unsigned b, unsigned n_init, const State &st,
+                                            const double *__restrict__ sums, size_t stride,
+                                            const int *__restrict__ poison, const Consts &c,
+                                            const Counters *cnt) {
+  const unsigned i = b * BSP + threadIdx.x;
   if (i >= cnt->N || i >= n_init) return;
   const uint32_t fl = st.flags[i];
   if (fl & FL_BLANKED) return;
@@ -727,6 +739,11 @@ __global__ void __launch_bounds__(BSP)
   const double thermal_alpha = tool_nz ? c.alpha_tool : c.alpha_wp;
   const bool brook = c.heat == MFG_HEAT_BROOKSHAW && thermal_alpha != 0.;
   particle_update(st, i, fl, c, ps, brook, thermal_alpha);
+}
+__global__ void __launch_bounds__(BSP)
+    k_particle_update(unsigned n_init, State st, const double *__restrict__ sums, size_t stride,
+                      const int *__restrict__ poison, Consts c, const Counters *cnt) {
+  update_body(blockIdx.x, n_init, st, sums, stride, poison, c, cnt);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1162,12 +1179,14 @@ __global__ void __launch_bounds__(BSP, PAIR64_WAVES)
 // FP32 list pass with the cooperative gather for the main sum: the 64 lanes of a wavefront run
 // the pair loop together (inactive lanes with an empty list) because every lane helps load.
 template <class Acc>
-__global__ void __launch_bounds__(BSP, PAIR32_WAVES)
-    k_pair32_coop(unsigned n_init, State st, const Rec32 *__restrict__ rec,
-                  const int *__restrict__ nbr, int stride, const int *__restrict__ count,
-                  Consts c, const Counters *cnt, double *__restrict__ sums, size_t sstride,
-                  const int *__restrict__ trav, const double4 *__restrict__ cs4) {
-  const unsigned t = xcd_block() * BSP + threadIdx.x;
+__device__ __forceinline__ void pair32_coop(unsigned b, unsigned nb, unsigned n_init,
+                                            const State &st, const Rec32 *__restrict__ rec,
+                                            const int *__restrict__ nbr, int stride,
+                                            const int *__restrict__ count, const Consts &c,
+                                            const Counters *cnt, double *__restrict__ sums,
+                                            size_t sstride, const int *__restrict__ trav,
+                                            const double4 *__restrict__ cs4) {
+  const unsigned t = xcd_map(b, nb) * BSP + threadIdx.x;
   const unsigned N = cnt->N;
   const bool in_range = t < N && t < n_init;
   const unsigned i = in_range && trav ? (unsigned)trav[t] : t;
@@ -1224,6 +1243,15 @@ __global__ void __launch_bounds__(BSP, PAIR32_WAVES)
   }
   if (!act) return;
   P.store(sums, sstride, i);
+}
+template <class Acc>
+__global__ void __launch_bounds__(BSP, PAIR32_WAVES)
+    k_pair32_coop(unsigned n_init, State st, const Rec32 *__restrict__ rec,
+                  const int *__restrict__ nbr, int stride, const int *__restrict__ count,
+                  Consts c, const Counters *cnt, double *__restrict__ sums, size_t sstride,
+                  const int *__restrict__ trav, const double4 *__restrict__ cs4) {
+  pair32_coop<Acc>(blockIdx.x, gridDim.x, n_init, st, rec, nbr, stride, count, c, cnt, sums,
+                   sstride, trav, cs4);
 }
 
 // FP64 list pass with a cooperative gather through LDS.  Gathered lane by lane, each 16-byte
@@ -1361,14 +1389,16 @@ __device__ __forceinline__ void dma_list_walk(const Rec64 *__restrict__ rec, con
   wave_sync_lds();  // the area is free for the next walk
 }
 
-__global__ void __launch_bounds__(BSP, PAIR64D_WAVES)
-    k_pair64_dma(unsigned n_init, State st, const Rec64 *__restrict__ rec,
-                 const int *__restrict__ nbr, int stride, const int *__restrict__ count,
-                 Consts c, const Counters *cnt, double *__restrict__ sums, size_t sstride,
-                 const int *__restrict__ trav, const double4 *__restrict__ cs4) {
+__device__ __forceinline__ void pair64_dma(unsigned b, unsigned nb, unsigned n_init,
+                                           const State &st, const Rec64 *__restrict__ rec,
+                                           const int *__restrict__ nbr, int stride,
+                                           const int *__restrict__ count, const Consts &c,
+                                           const Counters *cnt, double *__restrict__ sums,
+                                           size_t sstride, const int *__restrict__ trav,
+                                           const double4 *__restrict__ cs4) {
   typedef Pair64::CRec CRec;
   static_assert(sizeof(Rec64) == 192 && sizeof(CRec) == 64, "record pieces");
-  const unsigned t = xcd_block() * BSP + threadIdx.x;
+  const unsigned t = xcd_map(b, nb) * BSP + threadIdx.x;
   const unsigned N = cnt->N;
   const bool in_range = t < N && t < n_init;
   const unsigned i = in_range && trav ? (unsigned)trav[t] : t;
@@ -1394,6 +1424,14 @@ __global__ void __launch_bounds__(BSP, PAIR64D_WAVES)
                     [&](const auto &pc) { P.term(c, __builtin_bit_cast(Rec64, pc)); });
   if (!act) return;
   P.store(sums, sstride, i);
+}
+__global__ void __launch_bounds__(BSP, PAIR64D_WAVES)
+    k_pair64_dma(unsigned n_init, State st, const Rec64 *__restrict__ rec,
+                 const int *__restrict__ nbr, int stride, const int *__restrict__ count,
+                 Consts c, const Counters *cnt, double *__restrict__ sums, size_t sstride,
+                 const int *__restrict__ trav, const double4 *__restrict__ cs4) {
+  pair64_dma(blockIdx.x, gridDim.x, n_init, st, rec, nbr, stride, count, c, cnt, sums, sstride,
+             trav, cs4);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1676,6 +1714,66 @@ void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *re
     hipLaunchKernelGGL((k_pair32_coop<AccWide>), grid, block, 0, s, n_init, st,
                        (const Rec32 *)rec, nbr, nl_stride, count, c, cnt, sums, sstride, trav,
                        cs4);
+}
+
+// ---------------------------------------------------------------------------------------------
+// batched launches (after the permutation an instance's state is buf[cur ^ 1]): neighbour lists
+// (+ PSE), the pair sums in the list form and sorted visiting order, and the particle update
+// ---------------------------------------------------------------------------------------------
+// fewer waves per SIMD than the single-instance kernel (FP64 PSE 5, FP32 PSE 6, no PSE 7): the
+// instance lookup and descriptor take the registers that would otherwise spill
+template <int PSE_MODE>
+__global__ void __launch_bounds__(NB_BLOCK)
+    __attribute__((amdgpu_waves_per_eu(PSE_MODE == 1 ? NB_WAVES_FP64_PSE - 1
+                                       : PSE_MODE == 2 ? NB_WAVES_OTHER - 2
+                                                       : NB_WAVES_OTHER - 1)))
+    kb_cell_neighbours(const Inst *in, BatchMap m) {
+  unsigned b, nb;
+  const Inst &d = in[batch_locate(m, b, nb)];
+  const State &st = d.buf[d.cur ^ 1];
+  cell_neighbours<PSE_MODE>(b, nb, d.g, d.gb.count, d.nl4, d.pse2, st.h, st.flags, d.c, d.nbr,
+                            d.nl_stride, d.count, st.T_t, d.cnt, nullptr, d.poison);
+}
+
+__global__ void __launch_bounds__(BSP, PAIR64D_WAVES) kb_pair64_dma(const Inst *in, BatchMap m) {
+  unsigned b, nb;
+  const Inst &d = in[batch_locate(m, b, nb)];
+  pair64_dma(b, nb, d.n_init, d.buf[d.cur ^ 1], (const Rec64 *)d.rec, d.nbr, d.nl_stride, d.count,
+             d.c, d.cnt, d.sums, d.sum_stride, nullptr, d.cs4);
+}
+
+template <class Acc>
+__global__ void __launch_bounds__(BSP, PAIR32_WAVES) kb_pair32_coop(const Inst *in, BatchMap m) {
+  unsigned b, nb;
+  const Inst &d = in[batch_locate(m, b, nb)];
+  pair32_coop<Acc>(b, nb, d.n_init, d.buf[d.cur ^ 1], (const Rec32 *)d.rec, d.nbr, d.nl_stride,
+                   d.count, d.c, d.cnt, d.sums, d.sum_stride, nullptr, d.cs4);
+}
+
+__global__ void __launch_bounds__(BSP) kb_particle_update(const Inst *in, BatchMap m) {
+  unsigned b, nb;
+  const Inst &d = in[batch_locate(m, b, nb)];
+  update_body(b, d.n_init, d.buf[d.cur ^ 1], d.sums, d.sum_stride, d.poison, d.c, d.cnt);
+}
+
+// The kernel forms of the single-instance step's defaults (the FP64 cooperative gather when it
+// is compiled in, the FP32 cooperative gather) in the sorted visiting order; the PSE mode as
+// run_neighbours picks it.
+void launch_batch_pair(hipStream_t s, const BatchLaunch &L) {
+  const dim3 gn(L.nbrs8.blocks), bn(NB_BLOCK);
+  const int mode = !L.pse ? 0 : L.precision == MFG_FP64 ? 1 : 2;
+  if (mode == 0) hipLaunchKernelGGL(kb_cell_neighbours<0>, gn, bn, 0, s, L.inst, L.nbrs8.m);
+  else if (mode == 1) hipLaunchKernelGGL(kb_cell_neighbours<1>, gn, bn, 0, s, L.inst, L.nbrs8.m);
+  else hipLaunchKernelGGL(kb_cell_neighbours<2>, gn, bn, 0, s, L.inst, L.nbrs8.m);
+  const dim3 gp(L.parts8.blocks), bp(BSP);
+  static_assert(PAIR64_COOP, "the batched FP64 pair pass is the cooperative form");
+  if (L.precision == MFG_FP64)
+    hipLaunchKernelGGL(kb_pair64_dma, gp, bp, 0, s, L.inst, L.parts8.m);
+  else if (L.precision == MFG_FP32_KAHAN)
+    hipLaunchKernelGGL(kb_pair32_coop<AccKahan>, gp, bp, 0, s, L.inst, L.parts8.m);
+  else
+    hipLaunchKernelGGL(kb_pair32_coop<AccWide>, gp, bp, 0, s, L.inst, L.parts8.m);
+  hipLaunchKernelGGL(kb_particle_update, dim3(L.parts.blocks), dim3(BSP), 0, s, L.inst, L.parts.m);
 }
 
 int pair_order_segments(const GridInfo &g, int strip) {

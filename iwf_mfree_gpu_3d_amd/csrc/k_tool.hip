@@ -365,9 +365,10 @@ __global__ void k_reset_u32(unsigned *p) {
 
 // list every eligible particle inside the box of an active exhaustive tool (the same test as
 // k_contact_bc) and remember its entry
-__global__ void __launch_bounds__(BST)
-    k_contact_list(State st, unsigned n_init, ToolSet ts, const Counters *cnt, ContactSearch cs) {
-  const unsigned i = blockIdx.x * BST + threadIdx.x;
+__device__ __forceinline__ void contact_list(unsigned b, const State &st, unsigned n_init,
+                                             const ToolSet &ts, const Counters *cnt,
+                                             const ContactSearch &cs) {
+  const unsigned i = b * BST + threadIdx.x;
   if (i >= n_init || i >= cnt->N) return;
   const uint32_t fl = st.flags[i];
   if ((fl & FL_BLANKED) || (fl & FL_TOOL)) return;
@@ -385,15 +386,19 @@ __global__ void __launch_bounds__(BST)
     cs.slot[(size_t)k * n_init + i] = sl;  // -1: searched per thread by k_contact_bc
   }
 }
+__global__ void __launch_bounds__(BST)
+    k_contact_list(State st, unsigned n_init, ToolSet ts, const Counters *cnt, ContactSearch cs) {
+  contact_list(blockIdx.x, st, n_init, ts, cnt, cs);
+}
 
 // one wavefront per listed pair: lanes take the candidate triangles (the cell's list, or all
 // triangles outside the grid) in strides of 64; the smallest squared distance wins, the
 // smaller triangle index on ties -- the first minimum of the reference's ascending loop
-__global__ void __launch_bounds__(BST)
-    k_contact_search(State st, ToolSet ts, ContactSearch cs) {
+__device__ __forceinline__ void contact_search(unsigned b, unsigned nb, const State &st,
+                                               const ToolSet &ts, const ContactSearch &cs) {
   const int lane = threadIdx.x & 63;
-  const unsigned nw = gridDim.x * (BST / 64);
-  const unsigned w0 = blockIdx.x * (BST / 64) + (threadIdx.x >> 6);
+  const unsigned nw = nb * (BST / 64);
+  const unsigned w0 = b * (BST / 64) + (threadIdx.x >> 6);
   const unsigned n = min(*cs.list_n, cs.cap);
   for (unsigned e = w0; e < n; e += nw) {
     const int2 ent = cs.list[e];
@@ -447,6 +452,10 @@ __global__ void __launch_bounds__(BST)
     }
   }
 }
+__global__ void __launch_bounds__(BST)
+    k_contact_search(State st, ToolSet ts, ContactSearch cs) {
+  contact_search(blockIdx.x, gridDim.x, st, ts, cs);
+}
 
 // normal orientation test of the tool_3d constructor (host unhash: unsigned casts)
 __global__ void k_orient(const Tri *boxed, const int *cells, int nx, int ny, int nz, double b0,
@@ -479,6 +488,15 @@ __global__ void k_flip(Tri *tris, int n) {
   }
 }
 
+// triangle k at time t: its vertices at time 0 plus v t (update_tool, src/tool_3d_gpu.cu:74-99)
+__device__ __forceinline__ void move_tri(Tri *tris, const double *p_init, int k, double t,
+                                         const double v[3]) {
+#pragma unroll
+  for (int q = 0; q < 3; q++)
+#pragma unroll
+    for (int a = 0; a < 3; a++) tris[k].p[q][a] = p_init[9 * k + 3 * q + a] + v[a] * t;
+}
+
 // two triangle arrays in one launch: threads [0, n) move the first, [n, n + n2) the second
 __global__ void k_tool_update(Tri *tris, const double *p_init, int n, Tri *tris2,
                               const double *p_init2, int n2, double t, double v0, double v1,
@@ -491,18 +509,15 @@ __global__ void k_tool_update(Tri *tris, const double *p_init, int n, Tri *tris2
     p_init = p_init2;
   }
   const double v[3] = {v0, v1, v2};
-#pragma unroll
-  for (int q = 0; q < 3; q++)
-#pragma unroll
-    for (int a = 0; a < 3; a++) tris[k].p[q][a] = p_init[9 * k + 3 * q + a] + v[a] * t;
+  move_tri(tris, p_init, k, t, v);
 }
 
 // ---- contact, friction heat, boundary conditions, tool particle motion ------------------------
-__global__ void __launch_bounds__(BST)
-    k_contact_bc(State st, unsigned n_init, ToolSet ts, Consts c, const Counters *cnt,
-                 float *partials) {
+__device__ __forceinline__ void contact_bc(unsigned b, const State &st, unsigned n_init,
+                                           const ToolSet &ts, const Consts &c,
+                                           const Counters *cnt, float *partials) {
   __shared__ float red[MAX_TOOLS * 3][BST / 64];
-  const unsigned i = blockIdx.x * BST + threadIdx.x;
+  const unsigned i = b * BST + threadIdx.x;
   const unsigned N = cnt->N;
   float frec[MAX_TOOLS * 3];
   if (ts.record)
@@ -644,17 +659,26 @@ __global__ void __launch_bounds__(BST)
       for (int q = 0; q < ts.n * 3; q++) {
         float v = 0.f;
         for (int w = 0; w < BST / 64; w++) v += red[q][w];
-        partials[(size_t)blockIdx.x * MAX_TOOLS * 3 + q] = v;
+        partials[(size_t)b * MAX_TOOLS * 3 + q] = v;
       }
   }
 }
+__global__ void __launch_bounds__(BST)
+    k_contact_bc(State st, unsigned n_init, ToolSet ts, Consts c, const Counters *cnt,
+                 float *partials) {
+  contact_bc(blockIdx.x, st, n_init, ts, c, cnt, partials);
+}
 
-__global__ void k_force_reduce(const float *partials, int nblocks, int n_tools, float *forces) {
+__device__ __forceinline__ void force_reduce(const float *partials, int nblocks, int n_tools,
+                                             float *forces) {
   int q = threadIdx.x;
   if (q >= n_tools * 3) return;
   float s = 0.f;
   for (int b = 0; b < nblocks; b++) s += partials[(size_t)b * MAX_TOOLS * 3 + q];
   forces[q] += s;
+}
+__global__ void k_force_reduce(const float *partials, int nblocks, int n_tools, float *forces) {
+  force_reduce(partials, nblocks, n_tools, forces);
 }
 
 // ---- NaN scan (debug_check_valid_full) ---------------------------------------------------------
@@ -815,6 +839,88 @@ __global__ void k_zero_f32(float *a, unsigned n) {
 void launch_zero_f32(hipStream_t s, float *a, unsigned n) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_zero_f32, dim3(nbk(n)), dim3(BST), 0, s, a, n);
+}
+
+// ---------------------------------------------------------------------------------------------
+// batched launches: tool motion on the device clock, contact, and the end of the step
+// ---------------------------------------------------------------------------------------------
+// Workgroup k of an instance moves its tool k to the instance's time t and advances the tool's
+// moving box as mfg_tool_update does on the host (a tool without boxed triangles is left as it
+// is and inactive); workgroup 0 also clears the force record and the search list counter.
+__global__ void __launch_bounds__(BST) kb_tool_advance(Inst *in, BatchMap m, int record) {
+  unsigned b, nb;
+  Inst &d = in[batch_locate(m, b, nb)];
+  const double t = d.t, dt = d.c.dt;
+  if (b == 0) {
+    if (record && d.forces)
+      for (int q = threadIdx.x; q < 3 * d.n_forces; q += BST) d.forces[q] = 0.f;
+    if (d.search && threadIdx.x == 0) *d.cs.list_n = 0u;
+  }
+  if ((int)b >= d.ts.n) return;
+  ToolMotion &tm = d.tm[b];
+  ToolView &tv = d.ts.t[b];
+  if (tm.n_boxed == 0) {
+    if (threadIdx.x == 0) tv.active = 0;
+    return;
+  }
+  if (tm.algo != MFG_CONTACT_HASHED)
+    for (int k = threadIdx.x; k < tm.n_tris; k += BST) move_tri(tm.tris, tm.p_init, k, t, tm.vel);
+  for (int k = threadIdx.x; k < tm.n_boxed; k += BST)
+    move_tri(tm.boxed, tm.p_init_boxed, k, t, tm.vel);
+  if (threadIdx.x == 0) {
+    tm.mesh_tc = t;
+    for (int a = 0; a < 3; a++) {
+      tv.bbmin[a] += dt * tm.vel[a];
+      tv.bbmax[a] += dt * tm.vel[a];
+      tv.corg[a] = tm.gbbmin[a] + tm.vel[a] * t;
+    }
+    tv.active = (t >= tm.birth && t <= tm.death) ? 1 : 0;
+  }
+}
+
+__global__ void __launch_bounds__(BST) kb_contact_list(const Inst *in, BatchMap m) {
+  unsigned b, nb;
+  const Inst &d = in[batch_locate(m, b, nb)];
+  if (!d.search) return;
+  contact_list(b, d.buf[d.cur ^ 1], d.n_init, d.ts, d.cnt, d.cs);
+}
+
+__global__ void __launch_bounds__(BST) kb_contact_search(const Inst *in, BatchMap m) {
+  unsigned b, nb;
+  const Inst &d = in[batch_locate(m, b, nb)];
+  if (!d.search) return;
+  contact_search(b, nb, d.buf[d.cur ^ 1], d.ts, d.cs);
+}
+
+__global__ void __launch_bounds__(BST) kb_contact_bc(const Inst *in, BatchMap m) {
+  unsigned b, nb;
+  const Inst &d = in[batch_locate(m, b, nb)];
+  contact_bc(b, d.buf[d.cur ^ 1], d.n_init, d.ts, d.c, d.cnt, d.partials);
+}
+
+// one workgroup per instance: the force record's fixed-order sum, then the clock advances by
+// dt (the driver's global_time_current += dt) and the state buffers swap roles
+__global__ void __launch_bounds__(64) kb_step_end(Inst *in, BatchMap m, int record) {
+  unsigned b, nb;
+  Inst &d = in[batch_locate(m, b, nb)];
+  if (record && d.forces && d.ts.n > 0) force_reduce(d.partials, d.contact_blocks, d.ts.n, d.forces);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    d.t = d.t + d.c.dt;
+    d.cur ^= 1;
+  }
+}
+
+void launch_batch_contact(hipStream_t s, const BatchLaunch &L, int record) {
+  Inst *in = const_cast<Inst *>(L.inst);
+  hipLaunchKernelGGL(kb_tool_advance, dim3(L.tools.blocks), dim3(BST), 0, s, in, L.tools.m, record);
+  if (L.search.blocks > 0) {
+    hipLaunchKernelGGL(kb_contact_list, dim3(L.parts.blocks), dim3(BST), 0, s, L.inst, L.parts.m);
+    hipLaunchKernelGGL(kb_contact_search, dim3(L.search.blocks), dim3(BST), 0, s, L.inst,
+                       L.search.m);
+  }
+  hipLaunchKernelGGL(kb_contact_bc, dim3(L.parts.blocks), dim3(BST), 0, s, L.inst, L.parts.m);
+  hipLaunchKernelGGL(kb_step_end, dim3(L.one.blocks), dim3(64), 0, s, in, L.one.m, record);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -18,11 +18,28 @@ namespace mfg {
 #ifndef MFG_XCD_REMAP
 #define MFG_XCD_REMAP 1
 #endif
-__device__ __forceinline__ unsigned xcd_block() {
-  if (!MFG_XCD_REMAP) return blockIdx.x;
-  const unsigned nb = gridDim.x, b = blockIdx.x;
+// The same map for workgroup b of nb (a batched launch gives each instance a range of a multiple
+// of 8 workgroups, so b % 8 is still the XCD).
+__device__ __forceinline__ unsigned xcd_map(unsigned b, unsigned nb) {
+  if (!MFG_XCD_REMAP) return b;
   const unsigned x = b & 7u, q = nb >> 3, r = nb & 7u;
   return x * q + (x < r ? x : r) + (b >> 3);
+}
+__device__ __forceinline__ unsigned xcd_block() { return xcd_map(blockIdx.x, gridDim.x); }
+
+// Instance of this workgroup in a batched launch, its workgroup index b inside the instance's
+// range and the range's size nb: the last k with off[k] <= blockIdx.x (empty ranges are skipped).
+__device__ __forceinline__ int batch_locate(const BatchMap &m, unsigned &b, unsigned &nb) {
+  const unsigned g = blockIdx.x;
+  int lo = 0, hi = m.n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (m.off[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  b = g - m.off[lo];
+  nb = m.off[lo + 1] - m.off[lo];
+  return lo;
 }
 
 

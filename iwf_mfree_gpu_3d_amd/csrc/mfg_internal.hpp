@@ -234,6 +234,85 @@ void launch_force_reduce(hipStream_t s, const float *partials, int nblocks, int 
 // forces[0 .. n) = 0 by a kernel
 void launch_zero_f32(hipStream_t s, float *a, unsigned n);
 
+// ---------------------------------------------------------------------------------------------
+// Batched launches: many independent instances advance together, every kernel of the step
+// launched once over all of them.  Instance k owns the workgroups [off[k], off[k + 1]) of a
+// launch; a workgroup finds its instance by a binary search of `off` and takes its arguments from
+// the instance's descriptor.  Each kernel body is shared with the single-instance launch, so an
+// instance computes exactly what it computes alone.
+// ---------------------------------------------------------------------------------------------
+struct BatchMap {
+  const unsigned *off;  // n + 1 ascending workgroup offsets (device)
+  int n;
+};
+
+// tool motion of one instance (what mfg_tool_update does on the host every step)
+struct ToolMotion {
+  Tri *tris;                  // all triangles (moved for the exhaustive algorithm)
+  const double *p_init;
+  Tri *boxed;                 // per-cell copies
+  const double *p_init_boxed;
+  int n_tris, n_boxed, algo, pad;
+  double vel[3];
+  double gbbmin[3];           // triangle-grid origin at time 0
+  double mesh_tc;             // time the triangles were last moved to
+  double birth, death;
+};
+
+// descriptor of one instance in device memory; kernels of the batched step read it and the
+// step's last kernel advances the clock and flips the state buffers
+struct Inst {
+  State buf[2];
+  int cur;                    // buf[cur]: state at the start of the step
+  unsigned n_init;
+  GridInfo g;
+  GridBuffers gb;
+  int nparts;                 // scan tiles of the cell counts
+  int blanking;
+  double bb[6], vmax2;        // blanking box and squared speed limit
+  Consts c;
+  void *rec;
+  double4 *nl4;
+  double2 *pse2;
+  double4 *cs4;
+  int *nbr, *count, *poison;
+  int nl_stride, contact_blocks;
+  double *sums;
+  size_t sum_stride;
+  Counters *cnt;
+  float *partials;
+  float *forces;              // null: no force record
+  int n_forces, search;       // search: lane-parallel exhaustive contact search
+  ContactSearch cs;
+  ToolSet ts;
+  ToolMotion tm[MAX_TOOLS];
+  double t;                   // clock (global_time_current of the reference)
+};
+
+// host view of one batched launch shape: the device map and its total workgroups
+struct BatchGrid {
+  BatchMap m;
+  unsigned blocks;
+};
+struct BatchLaunch {
+  const Inst *inst;  // device array of n descriptors
+  int n;
+  int precision, pse;
+  BatchGrid cells1;  // num_cell + 1 entries, 256 per workgroup
+  BatchGrid cells;   // num_cell entries, 256 per workgroup
+  BatchGrid tiles;   // scan tiles of the cell counts
+  BatchGrid one;     // one workgroup per instance
+  BatchGrid parts;   // N_init particles, 256 per workgroup
+  BatchGrid parts8;  // the same, padded to a multiple of 8 workgroups (XCD-ordered kernels)
+  BatchGrid nbrs8;   // neighbour-pass workgroups, padded to a multiple of 8
+  BatchGrid tools;   // one workgroup per tool (at least one per instance)
+  BatchGrid search;  // lane-parallel contact search workgroups
+};
+void launch_batch_sort(hipStream_t s, const BatchLaunch &L);
+void launch_batch_permute(hipStream_t s, const BatchLaunch &L);
+void launch_batch_pair(hipStream_t s, const BatchLaunch &L);
+void launch_batch_contact(hipStream_t s, const BatchLaunch &L, int record);
+
 // triangle binning on the GPU (grid_cpu_tri::get_cells)
 int tool_bin_triangles(hipStream_t s, const Tri *tris, int n_tris, const double bbmin[3],
                        double dx, int nx, int ny, int nz, int **cells_out, int **tri_idx_out,

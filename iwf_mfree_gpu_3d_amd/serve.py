@@ -8,12 +8,18 @@ instances by continuous batching: every round advances each admitted instance by
 and a request that arrives meanwhile joins at the next round.  Each request's time to result is
 its completion time minus its arrival time.
 
-Instances share no device state.  The library's constants are process-wide (as the reference's
-are), so an instance re-installs its own constants, tools, blanking box and clock before each of
-its steps (after mfg_constants_reset); a result therefore does not depend on the mix of requests
-it ran with.  Each instance queues its work on one stream of a small pool (the least loaded at
-admission), so the kernels of different instances, whose grids are small, run on the GPU at the
-same time.  The batched form is the same loop with every request arriving at time zero.
+Instances share no device state.  Two ways to run them:
+- "batched" (default): the admitted instances of one precision form a batch (mfg_batch_*), so every
+  kernel of the step is launched once over all of them, each with its own grid, constants, tools
+  and clock; a round advances a batch by up to `steps_per_round` steps (no further than its
+  nearest finishing instance), optionally replayed from a captured graph.  Each precision's batch
+  has its own stream.  Instances join and leave between rounds.
+- "streams": each instance queues its own steps on one stream of a small pool (the least loaded at
+  admission).  The library's constants are process-wide (as the reference's are), so an instance
+  re-installs its own constants, tools, blanking box and clock before each of its steps.
+Either way an instance computes exactly what it computes alone, so a result does not depend on the
+mix of requests it ran with.  The batched configuration is the same loop with every request
+arriving at time zero.
 """
 from __future__ import annotations
 
@@ -178,20 +184,37 @@ class Instance:
         self.particles = self.grid = self.tools = self.blanking = self.stepper = None
 
 
+class _Group:
+    """The batch of one precision mode and its stream."""
+
+    def __init__(self, stream: int):
+        self.stream = stream
+        api.select_stream(stream)
+        self.batch = api.batch()
+        self.members = []  # Instance per batch slot
+
+
 def serve(requests: list, hbm_budget: float = 250e9, steps_per_round: int = 5, on_done=None,
-          clock=time.perf_counter, log=None, streams: int = 4, tuning=None, max_neighbours: int = 0) -> dict:
+          clock=time.perf_counter, log=None, streams: int = 4, tuning=None, max_neighbours: int = 0,
+          mode: str = "batched", use_graph: bool = True) -> dict:
     """Runs the request loop until every request has finished.  `on_done(instance)` is called
     for each instance that finished without an error, before its state is released; a request
     whose instance reports a device error gets that code as its record's "error" and no time to
     result (the error counters are sticky, so the check at the end covers every step).
     `max_neighbours`: neighbour-list capacity of the instances (0: the library default).
-    `streams`: size of the stream pool
-    the instances are spread over.  `tuning`: an autotune.Table whose per-bucket pair-pass
-    choice each instance runs with (None: the library defaults).  Returns per-request records
-    and p50 / p99 time to result and the aggregate particle-steps/s."""
+    `mode`: "batched" or "streams" (see the module notes); `use_graph`: batched rounds replay a
+    captured step.  `streams`: size of the stream pool the instances are spread over.  `tuning`
+    ("streams" only): an autotune.Table whose per-bucket pair-pass choice each instance runs with
+    (None: the library defaults).  Returns per-request records and p50 / p99 time to result and
+    the aggregate particle-steps/s."""
+    if mode not in ("batched", "streams"):
+        raise ValueError(f"mode {mode!r}")
+    if mode == "batched" and tuning is not None:
+        raise ValueError("a tuning table applies to the streams mode (batches use one pair-pass form)")
     _lib.init()
     api.stream_pool(streams)
     load = [0] * streams  # active instances per stream
+    groups = {}  # precision -> _Group (batched mode)
     pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
     active = []
     used = 0.0
@@ -205,11 +228,20 @@ def serve(requests: list, hbm_budget: float = 250e9, steps_per_round: int = 5, o
             if active and used + need > hbm_budget:
                 break  # first come, first served: later requests wait behind this one
             r = pending.pop(0)
-            k = min(range(streams), key=lambda q: load[q])
-            load[k] += 1
-            nx, ny, nz = r.lattice()
-            choice = tuning.get(nx * ny * nz, r.precision) if tuning is not None else (api.MFG_PAIR_LIST, 0)
-            inst = Instance(r, k, choice, max_neighbours=max_neighbours)
+            if mode == "batched":
+                grp = groups.get(r.precision)
+                if grp is None:
+                    grp = groups[r.precision] = _Group(len(groups) % streams)
+                inst = Instance(r, grp.stream, max_neighbours=max_neighbours)
+                grp.batch.add(inst.particles, inst.grid, inst.tools, inst.blanking, None, dt=inst.dt,
+                              t_current=inst.t)
+                grp.members.append(inst)
+            else:
+                k = min(range(streams), key=lambda q: load[q])
+                load[k] += 1
+                nx, ny, nz = r.lattice()
+                choice = tuning.get(nx * ny * nz, r.precision) if tuning is not None else (api.MFG_PAIR_LIST, 0)
+                inst = Instance(r, k, choice, max_neighbours=max_neighbours)
             used += need
             active.append(inst)
             records[r.rid] = {"rid": r.rid, "arrival": r.arrival, "admitted": clock() - t0, "particles": inst.n,
@@ -221,8 +253,20 @@ def serve(requests: list, hbm_budget: float = 250e9, steps_per_round: int = 5, o
             if pending:
                 time.sleep(max(0.0, min(0.05, pending[0].arrival - (clock() - t0))))
             continue
+        if mode == "batched":
+            for grp in groups.values():
+                if not grp.members:
+                    continue
+                api.select_stream(grp.stream)
+                k = min(steps_per_round, min(i.req.steps - i.k for i in grp.members))
+                grp.batch.step(k, use_graph=use_graph)
+                for i in grp.members:
+                    i.k += k
+                    steps_total += i.n * k
+        else:
+            for inst in active:
+                steps_total += inst.n * inst.advance(steps_per_round)
         for inst in list(active):
-            steps_total += inst.n * inst.advance(steps_per_round)
             if inst.done:
                 err = inst.finish()
                 rec = records[inst.req.rid]
@@ -235,8 +279,15 @@ def serve(requests: list, hbm_budget: float = 250e9, steps_per_round: int = 5, o
                     rec["error"] = int(err)
                     if log:
                         log(f"t={rec['finished']:8.2f}s request {inst.req.rid} failed with error {err}")
+                if mode == "batched":
+                    grp = groups[inst.req.precision]
+                    slot = grp.members.index(inst)
+                    api.select_stream(grp.stream)
+                    grp.batch.remove(slot)
+                    grp.members.pop(slot)
+                else:
+                    load[inst.stream] -= 1
                 inst.release()
-                load[inst.stream] -= 1
                 active.remove(inst)
                 used -= device_bytes(inst.req)
     api.device_synchronize()

@@ -36,6 +36,8 @@ class FakeInstance:
 
     def __init__(self, req, stream=0, choice=None, max_neighbours=0):
         self.req, self.k, self.n, self.stream, self.choice = req, 0, 1000, stream, choice
+        self.particles = self.grid = self.blanking = None
+        self.tools, self.dt, self.t = [], 1.0, 1.0
         FakeInstance.active += 1
         FakeInstance.peak = max(FakeInstance.peak, FakeInstance.active)
         FakeInstance.streams.append(stream)
@@ -56,8 +58,28 @@ class FakeInstance:
         FakeInstance.active -= 1
 
 
-def test_admission_respects_the_budget(monkeypatch):
+class FakeBatch:
+    """Stands in for mfg_batch: slots in order of addition, removal shifts later slots down."""
+    steps = []  # (size, steps) of every step call
+
+    def __init__(self):
+        self.slots = []
+
+    def add(self, particles, g, tools, bl, forces, dt=None, t_current=None):
+        self.slots.append(particles)
+        return len(self.slots) - 1
+
+    def remove(self, slot):
+        del self.slots[slot]
+
+    def step(self, n_steps=1, record_forces=False, use_graph=False):
+        FakeBatch.steps.append((len(self.slots), n_steps))
+
+
+@pytest.mark.parametrize("mode", ["batched", "streams"])
+def test_admission_respects_the_budget(monkeypatch, mode):
     monkeypatch.setattr(serve, "Instance", FakeInstance)
+    monkeypatch.setattr(api, "batch", FakeBatch)
     monkeypatch.setattr(_lib, "init", lambda *a, **k: None)
     for f in ("synchronize", "device_synchronize"):
         monkeypatch.setattr(api, f, lambda: None)
@@ -66,7 +88,8 @@ def test_admission_respects_the_budget(monkeypatch):
     reqs = serve.sweep_requests(count=12, batched=True, n_min=1e5, n_max=1e5, steps=(20, 20), seed=2)
     per = serve.device_bytes(reqs[0])
     FakeInstance.active = FakeInstance.peak = 0
-    res = serve.serve(reqs, hbm_budget=2.5 * per, steps_per_round=5)
+    FakeBatch.steps = []
+    res = serve.serve(reqs, hbm_budget=2.5 * per, steps_per_round=5, mode=mode)
     assert FakeInstance.peak == 2  # two fit the budget at a time
     assert len(res["requests"]) == 12 and all("time_to_result" in r for r in res["requests"])
     # first come, first served
@@ -75,13 +98,26 @@ def test_admission_respects_the_budget(monkeypatch):
     assert res["particle_steps"] == 12 * 20 * 1000
     # a request larger than the whole budget still runs when it is alone
     FakeInstance.active = FakeInstance.peak = 0
-    res = serve.serve(reqs[:2], hbm_budget=0.5 * per)
+    res = serve.serve(reqs[:2], hbm_budget=0.5 * per, mode=mode)
     assert FakeInstance.peak == 1 and len(res["requests"]) == 2
-    # instances admitted together go to different streams of the pool
     FakeInstance.active = FakeInstance.peak = 0
     FakeInstance.streams = []
-    serve.serve(reqs[:6], hbm_budget=10 * per, streams=4)
-    assert FakeInstance.streams == [0, 1, 2, 3, 0, 1]
+    if mode == "streams":
+        # instances admitted together go to different streams of the pool
+        serve.serve(reqs[:6], hbm_budget=10 * per, streams=4, mode=mode)
+        assert FakeInstance.streams == [0, 1, 2, 3, 0, 1]
+    else:
+        # one batch (and stream) per precision; rounds of 5 steps, never past an instance's last
+        for r, n in zip(reqs[:6], (20, 20, 7, 7, 13, 13)):
+            r.steps = n
+        FakeBatch.steps = []
+        serve.serve(reqs[:6], hbm_budget=10 * per, streams=4, mode=mode)
+        by_prec = {}
+        for r in reqs[:6]:
+            by_prec.setdefault(r.precision, len(by_prec))
+        assert FakeInstance.streams == [by_prec[r.precision] for r in reqs[:6]]
+        assert all(1 <= k <= 5 for _, k in FakeBatch.steps)
+        assert sum(size * k for size, k in FakeBatch.steps) == 2 * 20 + 2 * 7 + 2 * 13
 
 
 FIELDS = [_lib.F_POS, _lib.F_VEL, _lib.F_S, _lib.F_T, _lib.F_EPS_PL, _lib.F_RHO]
@@ -89,8 +125,10 @@ FIELDS = [_lib.F_POS, _lib.F_VEL, _lib.F_S, _lib.F_T, _lib.F_EPS_PL, _lib.F_RHO]
 
 @pytest.mark.gpu
 def test_result_same_alone_and_in_the_mix(hip_device):
-    """A request's fields after its last step are bit-identical whether it ran alone or
-    interleaved with others of different size, parameters and precision."""
+    """A request's fields after its last step are bit-identical whether it ran alone (its own
+    steps on one stream) or interleaved with others of different size, parameters and precision,
+    batched (one launch per kernel over the instances of a precision, steps replayed from a
+    graph) or on a stream pool."""
     reqs = serve.sweep_requests(count=3, batched=True, n_min=4e3, n_max=1.5e4, steps=(12, 12), seed=11,
                                 fp32_share=0.5)
     reqs[1].precision = _lib.MFG_FP64 if reqs[0].precision != _lib.MFG_FP64 else _lib.MFG_FP32_KAHAN
@@ -102,12 +140,15 @@ def test_result_same_alone_and_in_the_mix(hip_device):
             store[inst.req.rid] = {f: inst.particles.download(f)[order] for f in FIELDS}
         return on_done
 
-    alone, mix = {}, {}
-    serve.serve([reqs[0]], on_done=grab(alone), steps_per_round=4)
+    alone, mix, mix_streams = {}, {}, {}
+    serve.serve([reqs[0]], on_done=grab(alone), steps_per_round=4, mode="streams")
     res = serve.serve(reqs, on_done=grab(mix), steps_per_round=4)
+    serve.serve(reqs, on_done=grab(mix_streams), steps_per_round=4, mode="streams")
     assert sorted(mix) == [0, 1, 2] and len(res["requests"]) == 3
     for f in FIELDS:
         np.testing.assert_array_equal(mix[0][f], alone[0][f], err_msg=f"field {f}")
+        for rid in range(3):
+            np.testing.assert_array_equal(mix[rid][f], mix_streams[rid][f], err_msg=f"request {rid} field {f}")
     assert np.abs(alone[0][_lib.F_VEL]).max() > 0.0
 
 
@@ -185,8 +226,8 @@ def test_autotuned_batch_same_results(hip_device):
         return on_done
 
     plain, tuned = {}, {}
-    serve.serve(reqs, on_done=grab(plain), steps_per_round=4)
-    serve.serve(reqs, on_done=grab(tuned), steps_per_round=4, tuning=table)
+    serve.serve(reqs, on_done=grab(plain), steps_per_round=4, mode="streams")
+    serve.serve(reqs, on_done=grab(tuned), steps_per_round=4, tuning=table, mode="streams")
     for rid in plain:
         for f in FIELDS:
             np.testing.assert_array_equal(tuned[rid][f], plain[rid][f], err_msg=f"request {rid} field {f}")
