@@ -318,7 +318,7 @@ def sweep(args) -> None:
         log(f"autotune: {tune_s:.1f} s")
     mode = "streams" if (args.autotune or args.mode == "streams") else "batched"
     res = serve.serve(reqs, hbm_budget=args.hbm_budget_gb * 1e9, log=log, streams=args.streams,
-                      tuning=tuning, mode=mode, use_graph=not args.no_graph)
+                      tuning=tuning, mode=mode, use_graph=not args.no_graph, cell_tile=args.cell_tile)
     log(f"p50 {res['p50_s']:.2f} s, p99 {res['p99_s']:.2f} s, {res['particle_steps_per_s']:.4e} particle-steps/s "
         f"over {res['makespan_s']:.1f} s")
     out = {
@@ -329,9 +329,11 @@ def sweep(args) -> None:
         "data": "synthetic (seeded lattices + jitter, wedge tools)",
         "config": {"workload": workload, "requests": len(reqs), "hbm_budget_gb": args.hbm_budget_gb,
                    "streams": args.streams, "autotune": bool(args.autotune), "mode": mode,
-                   "graph": mode == "batched" and not args.no_graph},
+                   "graph": mode == "batched" and not args.no_graph, "cell_tile": args.cell_tile},
         "sweep": {"p50_time_to_result_s": res["p50_s"], "p99_time_to_result_s": res["p99_s"],
-                  "makespan_s": res["makespan_s"], "particle_steps": res["particle_steps"]},
+                  "makespan_s": res["makespan_s"], "particle_steps": res["particle_steps"],
+                  "setup_s": res["setup_s"],
+                  "particle_steps_per_s_excluding_setup": res["particle_steps"] / max(res["makespan_s"] - res["setup_s"], 1e-9)},
     }
     if tuning is not None:
         out["autotune"] = {"seconds": tune_s, "choices": [{"bucket_log2_particles": b, "precision": p,
@@ -380,6 +382,8 @@ def main():
     ap.add_argument("--mode", choices=["batched", "streams"], default="batched",
                     help="served / batched forms: one launch per kernel over the instances of a precision "
                          "(batched) or each instance's own kernels on a stream pool (streams)")
+    ap.add_argument("--cell-tile", type=int, default=0, choices=[0, 1, 2, 4, 8, 16],
+                    help="batched mode: cells per wavefront of the neighbour pass (0 automatic)")
     ap.add_argument("--no-graph", action="store_true",
                     help="batched mode: launch every step's kernels instead of replaying a captured step")
     ap.add_argument("--batched-instances", type=int, default=256,

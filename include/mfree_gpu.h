@@ -108,6 +108,10 @@ typedef struct {
                          16 (FP32 modes), or the sorted order when five x-rows of records fit in
                          2 MB (small instances: the strips would only add two launches), a
                          negative value the sorted order.  The sums do not depend on it. */
+  int cell_tile;      /* neighbour pass: consecutive grid cells each wavefront takes in turn (1, 2,
+                         4 or 8); 0 selects 4 where the grid has at least two cells per particle
+                         (wide margins around the block: an empty cell then costs two loads, not a
+                         workgroup) and 1 otherwise.  Lists and sums do not depend on it. */
 } mfg_options;
 enum { MFG_PAIR_LIST = 0, MFG_PAIR_TILED = 1 };
 
@@ -324,6 +328,10 @@ int mfg_batch_add(mfg_batch *b, mfg_particles *p, mfg_grid *g, mfg_tool *const *
                   double dt, double t_current, int *slot);
 int mfg_batch_remove(mfg_batch *b, int slot);
 int mfg_batch_size(const mfg_batch *b);
+/* cells per wavefront of the batch's neighbour pass (1, 2, 4, 8), as mfg_options.cell_tile for a
+ * single step; 0 (the default): mfg_options.cell_tile, automatic over the whole batch when that
+ * is 0 too.  Results do not depend on it. */
+int mfg_batch_set_cell_tile(mfg_batch *b, int cell_tile);
 int mfg_batch_get_time(const mfg_batch *b, int slot, double *t);
 int mfg_batch_step(mfg_batch *b, int n_steps, int record_forces, int use_graph);
 

@@ -60,7 +60,7 @@ class geom_constants(C.Structure):
 
 class options(C.Structure):
     _fields_ = [("cspm", C.c_int), ("heat", C.c_int), ("single_sqrt", C.c_int), ("max_neighbours", C.c_int),
-                ("pair_pass", C.c_int), ("pair_strip", C.c_int)]
+                ("pair_pass", C.c_int), ("pair_strip", C.c_int), ("cell_tile", C.c_int)]
 
 
 class blanking_state(C.Structure):
@@ -184,6 +184,7 @@ SIGNATURES = {
     "mfg_batch_add": (_i, [_vp, _vp, _vp, C.POINTER(_vp), _i, C.POINTER(blanking_state), _vp, _d, _d, _ip]),
     "mfg_batch_remove": (_i, [_vp, _i]),
     "mfg_batch_size": (_i, [_vp]),
+    "mfg_batch_set_cell_tile": (_i, [_vp, _i]),
     "mfg_batch_get_time": (_i, [_vp, _i, _dp]),
     "mfg_batch_step": (_i, [_vp, _i, _i, _i]),
     "mfg_debug_check_valid": (_i, [_vp]),

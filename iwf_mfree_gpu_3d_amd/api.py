@@ -88,13 +88,15 @@ MFG_PAIR_LIST, MFG_PAIR_TILED = 0, 1
 
 
 def set_options(cspm: bool = False, heat: int = MFG_HEAT_BROOKSHAW, single_sqrt: bool = True,
-                max_neighbours: int = 0, pair_pass: int = MFG_PAIR_LIST, pair_strip: int = 0) -> None:
+                max_neighbours: int = 0, pair_pass: int = MFG_PAIR_LIST, pair_strip: int = 0,
+                cell_tile: int = 0) -> None:
     """Switches that are compile-time macros in the reference (src/types.h:32-108), the
-    pair-pass kernel (list gather or cell-tiled: the same sums in the same order) and the list
-    pass's visiting order (strip width in grid columns; 0 default, negative the sorted order)."""
+    pair-pass kernel (list gather or cell-tiled: the same sums in the same order), the list
+    pass's visiting order (strip width in grid columns; 0 default, negative the sorted order) and
+    the neighbour pass's cell tile (cells per wavefront: 1, 2, 4, 8; 0 automatic)."""
     _lib.init()
     check(_L().mfg_set_options(C.byref(_lib.options(int(cspm), int(heat), int(single_sqrt), int(max_neighbours),
-                                                    int(pair_pass), int(pair_strip)))))
+                                                    int(pair_pass), int(pair_strip), int(cell_tile)))))
 
 
 def actions_setup_physical_constants(c: phys_constants) -> None:
@@ -530,12 +532,18 @@ class batch:
     instance (as a reference set-up does) right before `add`.  Results per instance are those of
     leap_frog.step on it alone."""
 
-    def __init__(self):
+    def __init__(self, cell_tile: int = 0):
         _lib.init()
         handle = C.c_void_p()
         check(_L().mfg_batch_create(C.byref(handle)))
         self._h = handle
         self._members = []  # the objects each instance uses stay alive while it is in the batch
+        if cell_tile:
+            self.set_cell_tile(cell_tile)
+
+    def set_cell_tile(self, cell_tile: int) -> None:
+        """Cells per wavefront of the neighbour pass (1, 2, 4, 8; 0 automatic)."""
+        check(_L().mfg_batch_set_cell_tile(self._h, int(cell_tile)))
 
     def __del__(self):
         h = getattr(self, "_h", None)

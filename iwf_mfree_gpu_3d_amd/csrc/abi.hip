@@ -47,7 +47,7 @@ struct Lib {
   hipStream_t streams[MFG_MAX_STREAMS] = {};
   int n_streams = 0;  // streams of the pool (mfg_stream_pool)
   int n_created = 0;  // streams created, >= n_streams
-  mfg_options opt{0, MFG_HEAT_BROOKSHAW, 1, 256, MFG_PAIR_LIST, 0};
+  mfg_options opt{0, MFG_HEAT_BROOKSHAW, 1, 256, MFG_PAIR_LIST, 0, 0};
   // exhaustive contact: closest triangle found by one wavefront per (particle, tool) pair;
   // MFG_LANE_CONTACT=0 in the environment selects the one-thread-per-particle search instead
   bool opt_contact_search = true;
@@ -443,15 +443,16 @@ int contact_search_buffers(mfg_particles *p, int n_tools, ContactSearch &cs) {
 // candidate / pair totals (a buffer sized on first use) for mfg_particle_gpu_pair_stats
 int run_neighbours(mfg_grid *g, mfg_particles *p, const Consts &c, int want_pse, bool stats = true) {
   const GridInfo gi = g->info();
+  const int tile = resolve_cell_tile(g_lib.opt.cell_tile, gi.num_cell, p->N_init);
   if (!stats) {
     p->stat_n = 0;
     launch_neighbours(g_lib.stream, gi, g->count, p->nl4, p->pse2, p->st().h, p->st().flags, c,
                       p->nbr, p->nl_stride, p->count, p->st().T_t, p->cnt, want_pse, nullptr,
-                      p->poison);
+                      p->poison, tile);
     HIP_TRY(hipGetLastError());
     return MFG_OK;
   }
-  const int nb = neighbour_blocks(gi);
+  const int nb = neighbour_blocks(gi, tile);
   if (nb > p->stat_cap) {
     if (p->stat_part) (void)hipFree(p->stat_part);
     p->stat_part = nullptr;
@@ -463,7 +464,7 @@ int run_neighbours(mfg_grid *g, mfg_particles *p, const Consts &c, int want_pse,
   p->stat_n = nb;
   launch_neighbours(g_lib.stream, gi, g->count, p->nl4, p->pse2, p->st().h, p->st().flags, c,
                     p->nbr, p->nl_stride, p->count, p->st().T_t, p->cnt, want_pse, p->stat_part,
-                    p->poison);
+                    p->poison, tile);
   HIP_TRY(hipGetLastError());
   return MFG_OK;
 }
@@ -701,6 +702,9 @@ int mfg_set_options(const mfg_options *o) {
   if (o->heat < MFG_HEAT_NONE || o->heat > MFG_HEAT_PSE) return fail(MFG_ERR_ARG, "heat model %d", o->heat);
   if (o->pair_pass != MFG_PAIR_LIST && o->pair_pass != MFG_PAIR_TILED)
     return fail(MFG_ERR_ARG, "pair pass %d", o->pair_pass);
+  if (o->cell_tile != 0 && o->cell_tile != 1 && o->cell_tile != 2 && o->cell_tile != 4 &&
+      o->cell_tile != 8 && o->cell_tile != 16)
+    return fail(MFG_ERR_ARG, "cell tile %d (0, 1, 2, 4, 8 or 16)", o->cell_tile);
   g_lib.opt = *o;
   if (g_lib.opt.max_neighbours <= 0) g_lib.opt.max_neighbours = 256;
   // list rows are read as 16-byte vectors: a multiple of 4 entries, at least 8
@@ -1902,6 +1906,7 @@ struct mfg_batch {
   };
   std::vector<Member> members;
   int precision = -1, pse = -1;
+  int cell_tile = 0;   // neighbour pass cells per wavefront; 0: the library option, else automatic
   bool dirty = true;   // the device descriptors and maps must be rebuilt before the next step
   int table_record = -1;
   Inst *d_inst = nullptr;
@@ -2047,6 +2052,14 @@ int batch_build(mfg_batch *b, int record) {
   std::vector<unsigned> maps(N_MAPS * map_len, 0u);
   BatchGrid *grids[N_MAPS] = {&b->L.cells1, &b->L.cells, &b->L.tiles, &b->L.one, &b->L.parts,
                               &b->L.parts8, &b->L.nbrs8, &b->L.tools, &b->L.search};
+  // cell tile of the batched neighbour pass: by the cells per particle of the whole batch
+  long cells = 0, parts = 0;
+  for (const mfg_batch::Member &m : b->members) {
+    cells += m.g->num_cell;
+    parts += m.p->N_init;
+  }
+  const int tile = resolve_cell_tile(b->cell_tile ? b->cell_tile : g_lib.opt.cell_tile, cells,
+                                     (unsigned)std::min<long>(parts, 0xffffffffL));
   for (int i = 0; i < n; i++) {
     const mfg_batch::Member &m = b->members[i];
     host[i] = batch_inst(m, record);
@@ -2059,7 +2072,7 @@ int batch_build(mfg_batch *b, int record) {
         1u,
         nblk,
         round8(nblk),
-        round8((unsigned)neighbour_blocks(gi)),
+        round8((unsigned)neighbour_blocks(gi, tile)),
         (unsigned)std::max<size_t>(1, m.tools.size()),
         m.search ? std::min(BATCH_SEARCH_BLOCKS, std::max(1u, nblk)) : 0u};
     for (int k = 0; k < N_MAPS; k++) maps[k * map_len + i + 1] = maps[k * map_len + i] + blocks[k];
@@ -2080,6 +2093,7 @@ int batch_build(mfg_batch *b, int record) {
   b->L.n = n;
   b->L.precision = b->precision;
   b->L.pse = b->pse;
+  b->L.cell_tile = tile;
   b->table_record = record;
   b->dirty = false;
   return MFG_OK;
@@ -2217,6 +2231,16 @@ int mfg_batch_remove(mfg_batch *b, int slot) {
 }
 
 int mfg_batch_size(const mfg_batch *b) { return b ? (int)b->members.size() : MFG_ERR_ARG; }
+
+int mfg_batch_set_cell_tile(mfg_batch *b, int cell_tile) {
+  if (!b) return fail(MFG_ERR_ARG, "null batch");
+  if (cell_tile != 0 && cell_tile != 1 && cell_tile != 2 && cell_tile != 4 && cell_tile != 8 &&
+      cell_tile != 16)
+    return fail(MFG_ERR_ARG, "cell tile %d (0, 1, 2, 4, 8 or 16)", cell_tile);
+  if (cell_tile != b->cell_tile) b->dirty = true;
+  b->cell_tile = cell_tile;
+  return MFG_OK;
+}
 
 int mfg_batch_get_time(const mfg_batch *b, int slot, double *t) {
   if (!b || !t || slot < 0 || slot >= (int)b->members.size())

@@ -361,8 +361,10 @@ __device__ __forceinline__ int range_of(const int *pref, int nr, int q) {
 #ifndef NB_WAVES_OTHER
 #define NB_WAVES_OTHER 8
 #endif
-// body of workgroup b of nb (the single-instance kernel and the batched one below)
-template <int PSE_MODE>
+// body of workgroup b of nb (the single-instance kernel and the batched one below); each wavefront
+// takes CPW consecutive cells in turn (the cell tile), so that an empty cell costs two loads
+// rather than a workgroup where the grid has wide margins around the block
+template <int PSE_MODE, int CPW>
 __device__ __forceinline__ void cell_neighbours(
     unsigned b, unsigned nb, const GridInfo &g, const int *__restrict__ off,
     const double4 *__restrict__ nl4, const double2 *__restrict__ pse2,
@@ -381,11 +383,10 @@ __device__ __forceinline__ void cell_neighbours(
     if (threadIdx.x < 64) s_exp[threadIdx.x] = EXP2_J64[threadIdx.x];
     __syncthreads();
   }
-  const long cell = (long)xcd_map(b, nb) * NB_WAVES + w;
+  const long c0 = ((long)xcd_map(b, nb) * NB_WAVES + w) * CPW;
   unsigned long long n_cand = 0, n_pair = 0;
-  const int s = cell < g.num_cell ? off[cell] : 0;
-  const int e = cell < g.num_cell ? off[cell + 1] : 0;
-  if (s < e) {
+  // the particles [s, e) of one cell against its stencil's candidates
+  auto do_cell = [&](const long cell, const int s, const int e) {
     const int nyz = g.ny * g.nz;
     const int gi = (int)(cell / nyz);
     const int gj = (int)((cell - (long)gi * nyz) / g.nz);
@@ -424,6 +425,23 @@ __device__ __forceinline__ void cell_neighbours(
     };
     cell_lists<PSE_MODE>(s, e, T, lane, s_pd[w], fetch, nl4, pse2, h, flags, c, nbr, cap, count,
                          T_t, cnt, poison, s_exp, n_cand, n_pair);
+  };
+  if constexpr (CPW == 1) {
+    const int s = c0 < g.num_cell ? off[c0] : 0;
+    const int e = c0 < g.num_cell ? off[c0 + 1] : 0;
+    if (s < e) do_cell(c0, s, e);
+  } else {
+    // the tile's cell offsets in one load, lane q holding off[c0 + q] (off[num_cell] = N), and
+    // the occupied cells as a ballot: empty cells cost nothing further
+    const int o = (lane <= CPW && c0 + lane <= g.num_cell) ? off[c0 + lane] : 0;
+    const int o_next = __shfl_down(o, 1, 64);
+    unsigned long long occ = __ballot(lane < CPW && c0 + lane < g.num_cell && o_next > o);
+    while (occ) {
+      const int q = __builtin_ctzll(occ);  // ascending cells, as the single-cell form visits them
+      occ &= occ - 1;
+      wave_lds_sync();  // the previous cell's lanes are done with this wave's LDS
+      do_cell(c0 + q, __builtin_amdgcn_readlane(o, q), __builtin_amdgcn_readlane(o, q + 1));
+    }
   }
   // candidate / pair totals of this workgroup (work accounting of the roofline figures); a null
   // stat_part skips them
@@ -443,16 +461,36 @@ __device__ __forceinline__ void cell_neighbours(
     stat_part[2 * b + 1] = bb;
   }
 }
-template <int PSE_MODE>
+// waves per SIMD: one fewer for a PSE sum over a tile of several cells (the tile's loop state
+// would otherwise spill)
+template <int PSE_MODE, int CPW>
 __global__ void __launch_bounds__(NB_BLOCK)
-    __attribute__((amdgpu_waves_per_eu(PSE_MODE == 1 ? NB_WAVES_FP64_PSE : NB_WAVES_OTHER)))
+    __attribute__((amdgpu_waves_per_eu((PSE_MODE == 1 ? NB_WAVES_FP64_PSE : NB_WAVES_OTHER) -
+                                       (PSE_MODE != 0 && CPW > 1 ? 2 : 0))))
     k_cell_neighbours(GridInfo g, const int *__restrict__ off, const double4 *__restrict__ nl4,
                       const double2 *__restrict__ pse2, const double *__restrict__ h,
                       const uint32_t *__restrict__ flags, Consts c, int *__restrict__ nbr, int cap,
                       int *__restrict__ count, double *__restrict__ T_t, Counters *cnt,
                       unsigned long long *__restrict__ stat_part, int *__restrict__ poison) {
-  cell_neighbours<PSE_MODE>(blockIdx.x, gridDim.x, g, off, nl4, pse2, h, flags, c, nbr, cap, count,
-                            T_t, cnt, stat_part, poison);
+  cell_neighbours<PSE_MODE, CPW>(blockIdx.x, gridDim.x, g, off, nl4, pse2, h, flags, c, nbr, cap,
+                                 count, T_t, cnt, stat_part, poison);
+}
+
+// launches kernel template K<PSE mode, cell tile> for run-time values of both
+template <template <int, int> class Launch, class... A>
+void dispatch_neighbours(int mode, int cpw, A &&...a) {
+#define MFG_NB_CPW(P)                                          \
+  switch (cpw) {                                               \
+    case 1: Launch<P, 1>::run(a...); break;                    \
+    case 2: Launch<P, 2>::run(a...); break;                    \
+    case 4: Launch<P, 4>::run(a...); break;                    \
+    case 8: Launch<P, 8>::run(a...); break;                    \
+    default: Launch<P, 16>::run(a...); break;                  \
+  }
+  if (mode == 0) { MFG_NB_CPW(0) }
+  else if (mode == 1) { MFG_NB_CPW(1) }
+  else { MFG_NB_CPW(2) }
+#undef MFG_NB_CPW
 }
 
 // Runs body(load(row[k])) for k = 0 .. n-1 in order.  The neighbour indices are read four at a
@@ -1663,20 +1701,35 @@ __global__ void __launch_bounds__(64, TILE_WAVES)
 // ---------------------------------------------------------------------------------------------
 static inline unsigned nb(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-int neighbour_blocks(const GridInfo &g) { return (int)((g.num_cell + NB_WAVES - 1) / NB_WAVES); }
+int neighbour_blocks(const GridInfo &g, int cell_tile) {
+  const long per = (long)NB_WAVES * cell_tile;
+  return (int)((g.num_cell + per - 1) / per);
+}
+
+int resolve_cell_tile(int option, long num_cell, unsigned n_init) {
+  if (option == 1 || option == 2 || option == 4 || option == 8 || option == 16) return option;
+  return num_cell >= 2 * (long)n_init ? 4 : 1;
+}
+
+template <int P, int CPW>
+struct SingleNeighbours {
+  static void run(hipStream_t s, const GridInfo &g, const int *offsets, const double4 *nl4,
+                  const double2 *pse2, const double *h, const uint32_t *flags, const Consts &c,
+                  int *nbr, int cap, int *count, double *T_t, Counters *cnt,
+                  unsigned long long *stat_part, int *poison) {
+    hipLaunchKernelGGL((k_cell_neighbours<P, CPW>), dim3(neighbour_blocks(g, CPW)),
+                       dim3(NB_BLOCK), 0, s, g, offsets, nl4, pse2, h, flags, c, nbr, cap, count,
+                       T_t, cnt, stat_part, poison);
+  }
+};
 
 void launch_neighbours(hipStream_t s, const GridInfo &g, const int *offsets, const double4 *nl4,
                        const double2 *pse2, const double *h, const uint32_t *flags,
                        const Consts &c, int *nbr, int cap, int *count, double *T_t, Counters *cnt,
-                       int want_pse, unsigned long long *stat_part, int *poison) {
-  const dim3 grid(neighbour_blocks(g)), block(NB_BLOCK);
-#define MFG_NB_LAUNCH(P)                                                                        \
-  hipLaunchKernelGGL((k_cell_neighbours<P>), grid, block, 0, s, g, offsets, nl4, pse2, h, flags, c, \
-                     nbr, cap, count, T_t, cnt, stat_part, poison)
-  if (!want_pse) MFG_NB_LAUNCH(0);
-  else if (c.precision == MFG_FP64) MFG_NB_LAUNCH(1);
-  else MFG_NB_LAUNCH(2);
-#undef MFG_NB_LAUNCH
+                       int want_pse, unsigned long long *stat_part, int *poison, int cell_tile) {
+  const int mode = !want_pse ? 0 : c.precision == MFG_FP64 ? 1 : 2;
+  dispatch_neighbours<SingleNeighbours>(mode, cell_tile, s, g, offsets, nl4, pse2, h, flags, c,
+                                        nbr, cap, count, T_t, cnt, stat_part, poison);
 }
 
 void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *rec, const int *nbr,
@@ -1722,18 +1775,28 @@ void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *re
 // ---------------------------------------------------------------------------------------------
 // fewer waves per SIMD than the single-instance kernel (FP64 PSE 5, FP32 PSE 6, no PSE 7): the
 // instance lookup and descriptor take the registers that would otherwise spill
-template <int PSE_MODE>
+template <int PSE_MODE, int CPW>
 __global__ void __launch_bounds__(NB_BLOCK)
-    __attribute__((amdgpu_waves_per_eu(PSE_MODE == 1 ? NB_WAVES_FP64_PSE - 1
-                                       : PSE_MODE == 2 ? NB_WAVES_OTHER - 2
-                                                       : NB_WAVES_OTHER - 1)))
+    __attribute__((amdgpu_waves_per_eu((PSE_MODE == 1 ? NB_WAVES_FP64_PSE - 1
+                                        : PSE_MODE == 2 ? NB_WAVES_OTHER - 2
+                                                        : NB_WAVES_OTHER - 1) -
+                                       (PSE_MODE != 0 && CPW > 1 ? 2 : 0))))
     kb_cell_neighbours(const Inst *in, BatchMap m) {
   unsigned b, nb;
   const Inst &d = in[batch_locate(m, b, nb)];
   const State &st = d.buf[d.cur ^ 1];
-  cell_neighbours<PSE_MODE>(b, nb, d.g, d.gb.count, d.nl4, d.pse2, st.h, st.flags, d.c, d.nbr,
-                            d.nl_stride, d.count, st.T_t, d.cnt, nullptr, d.poison);
+  cell_neighbours<PSE_MODE, CPW>(b, nb, d.g, d.gb.count, d.nl4, d.pse2, st.h, st.flags, d.c,
+                                 d.nbr, d.nl_stride, d.count, st.T_t, d.cnt, nullptr, d.poison);
 }
+
+
+template <int P, int CPW>
+struct BatchNeighbours {
+  static void run(hipStream_t s, const BatchLaunch &L) {
+    hipLaunchKernelGGL((kb_cell_neighbours<P, CPW>), dim3(L.nbrs8.blocks), dim3(NB_BLOCK), 0, s,
+                       L.inst, L.nbrs8.m);
+  }
+};
 
 __global__ void __launch_bounds__(BSP, PAIR64D_WAVES) kb_pair64_dma(const Inst *in, BatchMap m) {
   unsigned b, nb;
@@ -1760,11 +1823,8 @@ __global__ void __launch_bounds__(BSP) kb_particle_update(const Inst *in, BatchM
 // is compiled in, the FP32 cooperative gather) in the sorted visiting order; the PSE mode as
 // run_neighbours picks it.
 void launch_batch_pair(hipStream_t s, const BatchLaunch &L) {
-  const dim3 gn(L.nbrs8.blocks), bn(NB_BLOCK);
   const int mode = !L.pse ? 0 : L.precision == MFG_FP64 ? 1 : 2;
-  if (mode == 0) hipLaunchKernelGGL(kb_cell_neighbours<0>, gn, bn, 0, s, L.inst, L.nbrs8.m);
-  else if (mode == 1) hipLaunchKernelGGL(kb_cell_neighbours<1>, gn, bn, 0, s, L.inst, L.nbrs8.m);
-  else hipLaunchKernelGGL(kb_cell_neighbours<2>, gn, bn, 0, s, L.inst, L.nbrs8.m);
+  dispatch_neighbours<BatchNeighbours>(mode, L.cell_tile, s, L);
   const dim3 gp(L.parts8.blocks), bp(BSP);
   static_assert(PAIR64_COOP, "the batched FP64 pair pass is the cooperative form");
   if (L.precision == MFG_FP64)

@@ -185,11 +185,14 @@ void launch_bbox_reduce(hipStream_t s, const State &st, const Counters *cnt, uns
 // neighbour lists (+ PSE heat) for every cell; stat_part gets 2 totals (candidates, pairs) per
 // workgroup, neighbour_blocks(g) workgroups; poison[p] = 1 when a candidate of particle p holds a
 // non-finite record (nl4.w == 2), which the reference's sums carry into p's rates
-int neighbour_blocks(const GridInfo &g);
+// cell_tile: consecutive cells per wavefront (1, 2, 4 or 8; resolve_cell_tile turns the option,
+// 0 for automatic, into one of them)
+int neighbour_blocks(const GridInfo &g, int cell_tile);
+int resolve_cell_tile(int option, long num_cell, unsigned n_init);
 void launch_neighbours(hipStream_t s, const GridInfo &g, const int *offsets, const double4 *nl4,
                        const double2 *pse2, const double *h, const uint32_t *flags,
                        const Consts &c, int *nbr, int cap, int *count, double *T_t, Counters *cnt,
-                       int want_pse, unsigned long long *stat_part, int *poison);
+                       int want_pse, unsigned long long *stat_part, int *poison, int cell_tile);
 // pair pass (neighbour sums).  tiled: one wavefront per grid column stages the neighbour
 // records of the column's particles in LDS (offsets and sorted keys of the grid); otherwise one
 // thread per particle gathers them from memory
@@ -298,6 +301,7 @@ struct BatchLaunch {
   const Inst *inst;  // device array of n descriptors
   int n;
   int precision, pse;
+  int cell_tile;     // cells per wavefront of the neighbour pass
   BatchGrid cells1;  // num_cell + 1 entries, 256 per workgroup
   BatchGrid cells;   // num_cell entries, 256 per workgroup
   BatchGrid tiles;   // scan tiles of the cell counts

@@ -187,23 +187,24 @@ class Instance:
 class _Group:
     """The batch of one precision mode and its stream."""
 
-    def __init__(self, stream: int):
+    def __init__(self, stream: int, cell_tile: int = 0):
         self.stream = stream
         api.select_stream(stream)
-        self.batch = api.batch()
+        self.batch = api.batch(cell_tile)
         self.members = []  # Instance per batch slot
 
 
 def serve(requests: list, hbm_budget: float = 250e9, steps_per_round: int = 5, on_done=None,
           clock=time.perf_counter, log=None, streams: int = 4, tuning=None, max_neighbours: int = 0,
-          mode: str = "batched", use_graph: bool = True) -> dict:
+          mode: str = "batched", use_graph: bool = True, cell_tile: int = 0) -> dict:
     """Runs the request loop until every request has finished.  `on_done(instance)` is called
     for each instance that finished without an error, before its state is released; a request
     whose instance reports a device error gets that code as its record's "error" and no time to
     result (the error counters are sticky, so the check at the end covers every step).
     `max_neighbours`: neighbour-list capacity of the instances (0: the library default).
     `mode`: "batched" or "streams" (see the module notes); `use_graph`: batched rounds replay a
-    captured step.  `streams`: size of the stream pool the instances are spread over.  `tuning`
+    captured step; `cell_tile`: cells per wavefront of the batches' neighbour pass (0 automatic).
+    `streams`: size of the stream pool the instances are spread over.  `tuning`
     ("streams" only): an autotune.Table whose per-bucket pair-pass choice each instance runs with
     (None: the library defaults).  Returns per-request records and p50 / p99 time to result and
     the aggregate particle-steps/s."""
@@ -221,6 +222,7 @@ def serve(requests: list, hbm_budget: float = 250e9, steps_per_round: int = 5, o
     records = {}
     t0 = clock()
     steps_total = 0
+    setup_s = 0.0  # time spent setting up admitted instances (inside the makespan)
     while pending or active:
         now = clock() - t0
         while pending and pending[0].arrival <= now:
@@ -228,10 +230,11 @@ def serve(requests: list, hbm_budget: float = 250e9, steps_per_round: int = 5, o
             if active and used + need > hbm_budget:
                 break  # first come, first served: later requests wait behind this one
             r = pending.pop(0)
+            ts = clock()
             if mode == "batched":
                 grp = groups.get(r.precision)
                 if grp is None:
-                    grp = groups[r.precision] = _Group(len(groups) % streams)
+                    grp = groups[r.precision] = _Group(len(groups) % streams, cell_tile)
                 inst = Instance(r, grp.stream, max_neighbours=max_neighbours)
                 grp.batch.add(inst.particles, inst.grid, inst.tools, inst.blanking, None, dt=inst.dt,
                               t_current=inst.t)
@@ -243,6 +246,7 @@ def serve(requests: list, hbm_budget: float = 250e9, steps_per_round: int = 5, o
                 choice = tuning.get(nx * ny * nz, r.precision) if tuning is not None else (api.MFG_PAIR_LIST, 0)
                 inst = Instance(r, k, choice, max_neighbours=max_neighbours)
             used += need
+            setup_s += clock() - ts
             active.append(inst)
             records[r.rid] = {"rid": r.rid, "arrival": r.arrival, "admitted": clock() - t0, "particles": inst.n,
                               "steps": r.steps, "precision": r.precision}
@@ -297,5 +301,5 @@ def serve(requests: list, hbm_budget: float = 250e9, steps_per_round: int = 5, o
     return {"requests": [records[k] for k in sorted(records)],
             "p50_s": float(np.percentile(ttr, 50)) if ttr.size else 0.0,
             "p99_s": float(np.percentile(ttr, 99)) if ttr.size else 0.0,
-            "particle_steps": steps_total, "makespan_s": makespan,
+            "particle_steps": steps_total, "makespan_s": makespan, "setup_s": setup_s,
             "particle_steps_per_s": steps_total / makespan if makespan > 0 else 0.0}

@@ -8,6 +8,9 @@
   columns into several chunks there.
 - The list pass's visiting order (strips of grid columns) against the sorted order: the order
   only decides which thread takes which particle, so every precision is bit-identical.
+- The neighbour pass's cell tile (cells per wavefront) against one cell per wavefront: each cell
+  is still one wavefront's work in the same order, so lists, PSE sums and every field are
+  bit-identical in every heat mode (PSE in FP64 and FP32, Brookshaw).
 """
 import numpy as np
 import pytest
@@ -20,9 +23,10 @@ FIELDS = [_lib.F_POS, _lib.F_VEL, _lib.F_RHO, _lib.F_S, _lib.F_T, _lib.F_EPS_PL,
           _lib.F_VEL_T, _lib.F_RHO_T, _lib.F_S_T, _lib.F_T_T, _lib.F_FC]
 
 
-def run_case(preset, scale, seed, precision, pair_pass, steps, pair_strip=0):
+def run_case(preset, scale, seed, precision, pair_pass, steps, pair_strip=0, cell_tile=0):
     sc = scenes.Scene(preset, scale, seed)
-    parts, grid = scenes.setup_cut(sc, precision=precision, pair_pass=pair_pass, pair_strip=pair_strip)
+    parts, grid = scenes.setup_cut(sc, precision=precision, pair_pass=pair_pass, pair_strip=pair_strip,
+                                   cell_tile=cell_tile)
     api.global_time_current = sc.params.dt
     lf = api.leap_frog(sc.n, grid.num_cell())
     scenes.run(lf, parts, grid, steps)
@@ -68,6 +72,19 @@ def test_visiting_order_does_not_change_results(hip_device, precision, strip):
     """Strips of 3 and 16 grid columns against the sorted order, thermal-contact grain, 10 steps."""
     a = run_case(2, 0.2, 3, precision, api.MFG_PAIR_LIST, 10, pair_strip=-1)
     b = run_case(2, 0.2, 3, precision, api.MFG_PAIR_LIST, 10, pair_strip=strip)
+    assert a["N"] == b["N"]
+    for f in FIELDS:
+        np.testing.assert_array_equal(b[f], a[f], err_msg=f"field {f}")
+
+
+@pytest.mark.parametrize("case", [(2, 0.2, 3, _lib.MFG_FP64), (2, 0.2, 3, _lib.MFG_FP32_KAHAN),
+                                  (1, 1.0, 1, _lib.MFG_FP64)], ids=["grain_fp64_pse", "grain_fp32_pse", "wedge_fp64"])
+@pytest.mark.parametrize("tile", [2, 4, 8])
+def test_cell_tile_does_not_change_results(hip_device, case, tile):
+    """Tiles of 2, 4 and 8 cells per wavefront against one, 30 steps: bit-identical."""
+    preset, scale, seed, precision = case
+    a = run_case(preset, scale, seed, precision, api.MFG_PAIR_LIST, 30, cell_tile=1)
+    b = run_case(preset, scale, seed, precision, api.MFG_PAIR_LIST, 30, cell_tile=tile)
     assert a["N"] == b["N"]
     for f in FIELDS:
         np.testing.assert_array_equal(b[f], a[f], err_msg=f"field {f}")

@@ -62,7 +62,7 @@ class FakeBatch:
     """Stands in for mfg_batch: slots in order of addition, removal shifts later slots down."""
     steps = []  # (size, steps) of every step call
 
-    def __init__(self):
+    def __init__(self, cell_tile=0):
         self.slots = []
 
     def add(self, particles, g, tools, bl, forces, dt=None, t_current=None):
