@@ -73,7 +73,7 @@ def gpu_run(args, preset=None, scale=None, precision=None, steps=None, warmup=No
         f"({scene.params.n_workpiece} workpiece), {sum(t[1].shape[0] for t in scene.tools)} tool triangles, "
         f"built in {time.time() - t0:.1f}s")
     prec = {"fp64": _lib.MFG_FP64, "fp32-kahan": _lib.MFG_FP32_KAHAN, "fp32-wide": _lib.MFG_FP32_WIDE}[precision]
-    parts, grid = scenes.setup_cut(scene, precision=prec, single_sqrt=True)
+    parts, grid = scenes.setup_cut(scene, precision=prec, single_sqrt=True, cell_tile=args.cell_tile)
     lf = api.leap_frog(scene.n, grid.num_cell())
     api.global_time_current = api.global_time_dt
     scenes.run(lf, parts, grid, warmup)
@@ -309,14 +309,15 @@ def sweep(args) -> None:
         workload = (f"served: Poisson arrivals 4/s over {args.served:g} s, N log-uniform in [2e4, 2e6] (4:2:1 blocks), "
                     f"70% FP32+Kahan / 30% FP64, 100-1000 steps, parameters as the batched form")
     log(f"{len(reqs)} requests, {sum(r.lattice()[0] * r.lattice()[1] * r.lattice()[2] for r in reqs)} particles in all")
+    mode = args.mode
     tuning, tune_s = None, None
     if args.autotune:
         from iwf_mfree_gpu_3d_amd import autotune
         t0 = time.perf_counter()
-        tuning = autotune.tune(reqs, log=log)  # before the timed request loop
+        # before the timed request loop
+        tuning = autotune.tune(reqs, log=log) if mode == "streams" else autotune.tune_batched(reqs, log=log)
         tune_s = time.perf_counter() - t0
         log(f"autotune: {tune_s:.1f} s")
-    mode = "streams" if (args.autotune or args.mode == "streams") else "batched"
     res = serve.serve(reqs, hbm_budget=args.hbm_budget_gb * 1e9, log=log, streams=args.streams,
                       tuning=tuning, mode=mode, use_graph=not args.no_graph, cell_tile=args.cell_tile)
     log(f"p50 {res['p50_s']:.2f} s, p99 {res['p99_s']:.2f} s, {res['particle_steps_per_s']:.4e} particle-steps/s "
@@ -336,9 +337,11 @@ def sweep(args) -> None:
                   "particle_steps_per_s_excluding_setup": res["particle_steps"] / max(res["makespan_s"] - res["setup_s"], 1e-9)},
     }
     if tuning is not None:
-        out["autotune"] = {"seconds": tune_s, "choices": [{"bucket_log2_particles": b, "precision": p,
-                                                           "pair_pass": c[0], "strip": c[1]}
-                                                          for (b, p), c in sorted(tuning.choice.items())]}
+        out["autotune"] = {"seconds": tune_s,
+                           "choices": [{"bucket_log2_particles": b, "precision": p, "pair_pass": c[0],
+                                        "strip": c[1], "cell_tile": c[2]} for (b, p), c in sorted(tuning.choice.items())],
+                           "batch_cell_tile": {str(p): t for p, t in sorted(tuning.batch_tile.items())},
+                           "batch_ms_per_step": {f"{p}/{t}": s * 1e3 for (p, t), s in sorted(tuning.batch_times.items())}}
     print(json.dumps(out))
 
 
@@ -383,7 +386,7 @@ def main():
                     help="served / batched forms: one launch per kernel over the instances of a precision "
                          "(batched) or each instance's own kernels on a stream pool (streams)")
     ap.add_argument("--cell-tile", type=int, default=0, choices=[0, 1, 2, 4, 8, 16],
-                    help="batched mode: cells per wavefront of the neighbour pass (0 automatic)")
+                    help="cells per wavefront of the neighbour pass (0 automatic)")
     ap.add_argument("--no-graph", action="store_true",
                     help="batched mode: launch every step's kernels instead of replaying a captured step")
     ap.add_argument("--batched-instances", type=int, default=256,

@@ -109,9 +109,10 @@ typedef struct {
                          2 MB (small instances: the strips would only add two launches), a
                          negative value the sorted order.  The sums do not depend on it. */
   int cell_tile;      /* neighbour pass: consecutive grid cells each wavefront takes in turn (1, 2,
-                         4 or 8); 0 selects 4 where the grid has at least two cells per particle
-                         (wide margins around the block: an empty cell then costs two loads, not a
-                         workgroup) and 1 otherwise.  Lists and sums do not depend on it. */
+                         4, 8 or 16; the tile's empty cells cost one ballot, not a workgroup);
+                         0 selects 1, or, where the grid has two or more cells per particle
+                         (wide margins around the block), 4 from 256k cells and 16 from 1M.
+                         Lists and sums do not depend on it. */
 } mfg_options;
 enum { MFG_PAIR_LIST = 0, MFG_PAIR_TILED = 1 };
 
@@ -328,7 +329,7 @@ int mfg_batch_add(mfg_batch *b, mfg_particles *p, mfg_grid *g, mfg_tool *const *
                   double dt, double t_current, int *slot);
 int mfg_batch_remove(mfg_batch *b, int slot);
 int mfg_batch_size(const mfg_batch *b);
-/* cells per wavefront of the batch's neighbour pass (1, 2, 4, 8), as mfg_options.cell_tile for a
+/* cells per wavefront of the batch's neighbour pass (1, 2, 4, 8, 16), as mfg_options.cell_tile for a
  * single step; 0 (the default): mfg_options.cell_tile, automatic over the whole batch when that
  * is 0 too.  Results do not depend on it. */
 int mfg_batch_set_cell_tile(mfg_batch *b, int cell_tile);

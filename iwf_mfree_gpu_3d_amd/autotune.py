@@ -1,15 +1,23 @@
-"""Autotuner of the pair pass's run-time choices, per size bucket (batched and served forms).
+"""Autotuner of the step's launch choices, per size bucket (batched and served forms).
 
-The choices are the pair-pass form (MFG_PAIR_LIST, one thread per particle, or MFG_PAIR_TILED,
-one wavefront per grid column) and the list pass's visiting order (strip width in grid columns,
-or the sorted order).  Only choices that leave results bit-identical are offered: the visiting
-order in every precision, the tiled form in FP64 only (both forms form the same sums in the same
-order, but in the FP32 modes the two kernels' FMA contraction can differ at rounding level).  So
-the tuner only measures time.
+Every choice offered leaves results bit-identical, so the tuner only measures time:
+- the pair pass's form (MFG_PAIR_LIST, one thread per particle, or MFG_PAIR_TILED, one
+  wavefront per grid column) and the list pass's visiting order (strip width in grid columns, or
+  the sorted order).  The tiled form is offered in FP64 only: both forms form the same sums in
+  the same order, but in the FP32 modes the two kernels' FMA contraction can differ at rounding
+  level;
+- the neighbour pass's cell tile (consecutive grid cells per wavefront: 1, 2, 4, 8, 16), which
+  decides how much an empty cell of a grid with wide margins costs.
+
+For instances on a stream pool (serve mode "streams"), `tune` picks per (bucket, precision) the
+form and order first, then the cell tile with them.  For batches (serve mode "batched") the cell
+tile is a property of the whole launch: `tune_batched` measures a throwaway batch of the first
+requests of each precision with every tile.
 
 Not a choice here: the re-sort interval.  The reference rebuilds its cell lists from a fresh
-sort every step (src/leap_frog.cu:139-145), and sorting less often would change which
-particles are neighbours.  Block and cell-tile sizes are compile-time constants of the kernels.
+sort every step (src/leap_frog.cu:139-145); sorting less often would change which particles are
+neighbours, and the lists are to stay bit-exact.  Workgroup sizes and waves per SIMD are
+compile-time constants of the kernels (the measured alternatives are documented with them).
 
 Buckets are powers of two of the particle count; an instance takes the choice of its own
 bucket, else of the nearest tuned one, else the library default.
@@ -26,7 +34,9 @@ from . import api
 # (pair pass, strip): 0 the library's default, -1 the sorted order
 CANDIDATES = ((api.MFG_PAIR_LIST, 0), (api.MFG_PAIR_LIST, -1), (api.MFG_PAIR_LIST, 8),
               (api.MFG_PAIR_LIST, 16), (api.MFG_PAIR_TILED, 0))
-DEFAULT = (api.MFG_PAIR_LIST, 0)
+# cells per wavefront of the neighbour pass; 0 the library's automatic choice
+TILES = (1, 2, 4, 8, 16)
+DEFAULT = (api.MFG_PAIR_LIST, 0, 0)
 
 
 def candidates_for(precision: int, candidates=CANDIDATES) -> tuple:
@@ -39,11 +49,14 @@ def bucket(n: int) -> int:
 
 
 class Table:
-    """(bucket, precision) -> (pair pass, strip), with the measured seconds per step."""
+    """(bucket, precision) -> (pair pass, strip, cell tile), with the measured seconds per step;
+    precision -> cell tile of a batch."""
 
     def __init__(self):
         self.choice = {}
         self.times = {}
+        self.batch_tile = {}
+        self.batch_times = {}
 
     def get(self, n: int, precision: int) -> tuple:
         b = bucket(n)
@@ -54,17 +67,23 @@ class Table:
         return self.choice[key]
 
     def to_json(self) -> str:
-        return json.dumps({"choice": [[b, p, c[0], c[1]] for (b, p), c in sorted(self.choice.items())],
-                           "times": [[b, p, c[0], c[1], t] for (b, p, c), t in sorted(self.times.items())]})
+        return json.dumps({"choice": [[b, p, *c] for (b, p), c in sorted(self.choice.items())],
+                           "times": [[b, p, *c, t] for (b, p, c), t in sorted(self.times.items())],
+                           "batch_tile": [[p, t] for p, t in sorted(self.batch_tile.items())],
+                           "batch_times": [[p, t, s] for (p, t), s in sorted(self.batch_times.items())]})
 
     @staticmethod
     def from_json(text: str) -> "Table":
         d = json.loads(text)
         t = Table()
-        for b, p, pp, st in d["choice"]:
-            t.choice[(b, p)] = (pp, st)
-        for b, p, pp, st, s in d.get("times", []):
-            t.times[(b, p, (pp, st))] = s
+        for row in d["choice"]:
+            t.choice[(row[0], row[1])] = tuple(row[2:])
+        for row in d.get("times", []):
+            t.times[(row[0], row[1], tuple(row[2:-1]))] = row[-1]
+        for p, tile in d.get("batch_tile", []):
+            t.batch_tile[p] = tile
+        for p, tile, s in d.get("batch_times", []):
+            t.batch_times[(p, tile)] = s
         return t
 
     def save(self, path: str) -> None:
@@ -97,11 +116,11 @@ def measure(req, candidate: tuple, steps: int = 6, warmup: int = 2) -> float:
         gc.collect()
 
 
-def tune(requests, candidates=CANDIDATES, steps: int = 6, warmup: int = 2, measure=measure,
+def tune(requests, candidates=CANDIDATES, tiles=TILES, steps: int = 6, warmup: int = 2, measure=measure,
          log=None) -> Table:
-    """One representative request per (bucket, precision) among `requests` (the first seen),
-    each candidate measured on it; the fastest is the bucket's choice (ties: the earlier
-    candidate)."""
+    """One representative request per (bucket, precision) among `requests` (the first seen):
+    every form / order candidate measured on it with the automatic cell tile, then every cell
+    tile with the fastest of those; ties go to the earlier candidate."""
     table = Table()
     reps = {}
     for r in requests:
@@ -110,12 +129,65 @@ def tune(requests, candidates=CANDIDATES, steps: int = 6, warmup: int = 2, measu
     for key, r in sorted(reps.items()):
         best = None
         for cand in candidates_for(key[1], candidates):
-            s = measure(r, cand, steps=steps, warmup=warmup)
-            table.times[(key[0], key[1], tuple(cand))] = s
+            full = (cand[0], cand[1], 0)
+            s = measure(r, full, steps=steps, warmup=warmup)
+            table.times[(key[0], key[1], full)] = s
             if best is None or s < best[0]:
-                best = (s, tuple(cand))
+                best = (s, full)
+        for tile in tiles:
+            full = (best[1][0], best[1][1], tile)
+            s = measure(r, full, steps=steps, warmup=warmup)
+            table.times[(key[0], key[1], full)] = s
+            if s < best[0]:
+                best = (s, full)
         table.choice[key] = best[1]
         if log:
-            log(f"autotune bucket 2^{key[0]} precision {key[1]}: pair pass {best[1][0]}, strip {best[1][1]} "
-                f"({best[0] * 1e3:.3f} ms/step)")
+            log(f"autotune bucket 2^{key[0]} precision {key[1]}: pair pass {best[1][0]}, strip {best[1][1]}, "
+                f"cell tile {best[1][2]} ({best[0] * 1e3:.3f} ms/step)")
+    return table
+
+
+def measure_batch(requests, tile: int, steps: int = 4, warmup: int = 2) -> float:
+    """Seconds per batched step of fresh instances of `requests` with cell tile `tile`."""
+    from .serve import Instance
+    b = api.batch(tile)
+    insts = []
+    try:
+        for r in requests:
+            inst = Instance(r, 0)
+            b.add(inst.particles, inst.grid, inst.tools, inst.blanking, None, dt=inst.dt, t_current=inst.t)
+            insts.append(inst)
+        b.step(warmup)
+        api.synchronize()
+        t0 = time.perf_counter()
+        b.step(steps)
+        api.synchronize()
+        return (time.perf_counter() - t0) / steps
+    finally:
+        del b
+        for inst in insts:
+            inst.release()
+        del insts
+        gc.collect()
+
+
+def tune_batched(requests, tiles=TILES, sample: int = 16, steps: int = 4, warmup: int = 2,
+                 measure=measure_batch, log=None, table: Table | None = None) -> Table:
+    """The cell tile of each precision's batch: the first `sample` requests of the precision as a
+    throwaway batch, measured with every tile (ties: the smaller tile)."""
+    table = table or Table()
+    by_prec = {}
+    for r in requests:
+        by_prec.setdefault(r.precision, []).append(r)
+    for prec, reqs in sorted(by_prec.items()):
+        best = None
+        for tile in tiles:
+            s = measure(reqs[:sample], tile, steps=steps, warmup=warmup)
+            table.batch_times[(prec, tile)] = s
+            if best is None or s < best[0]:
+                best = (s, tile)
+        table.batch_tile[prec] = best[1]
+        if log:
+            log(f"autotune batch precision {prec}: cell tile {best[1]} ({best[0] * 1e3:.3f} ms per batched step of "
+                f"{min(sample, len(reqs))} instances)")
     return table

@@ -1706,9 +1706,14 @@ int neighbour_blocks(const GridInfo &g, int cell_tile) {
   return (int)((g.num_cell + per - 1) / per);
 }
 
+// Automatic: one cell per wavefront unless the grid is both sparse (two or more cells per
+// particle) and large, where empty cells would dominate the launch: 4 cells from 256k cells, 16
+// from 1M (measured: a single 8k-particle wedge cut or a 30k grain runs fastest with one cell
+// per wavefront, a batch of 256 wedge cuts (44M cells, 3.25 per particle) with 16)
 int resolve_cell_tile(int option, long num_cell, unsigned n_init) {
   if (option == 1 || option == 2 || option == 4 || option == 8 || option == 16) return option;
-  return num_cell >= 2 * (long)n_init ? 4 : 1;
+  if (num_cell < 2 * (long)n_init || num_cell < (1L << 18)) return 1;
+  return num_cell >= (1L << 20) ? 16 : 4;
 }
 
 template <int P, int CPW>

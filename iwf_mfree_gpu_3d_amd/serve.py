@@ -103,11 +103,12 @@ class Instance:
 
     DX = 5e-4  # lattice spacing [cm]
 
-    def __init__(self, req: Request, stream: int = 0, choice: tuple = (api.MFG_PAIR_LIST, 0),
+    def __init__(self, req: Request, stream: int = 0, choice: tuple = (api.MFG_PAIR_LIST, 0, 0),
                  max_neighbours: int = 0):
         self.req = req
         self.stream = stream
-        self.choice = tuple(choice)  # (pair pass, strip) the instance's steps run with
+        # (pair pass, strip, cell tile) the instance's steps run with
+        self.choice = (tuple(choice) + (0, 0, 0))[:3]
         api.select_stream(stream)
         nx, ny, nz = req.lattice()
         cfg = _lib.cut_config()
@@ -131,7 +132,8 @@ class Instance:
         p = self.scene.params
         api.constants_reset()
         api.set_options(cspm=bool(p.cspm), heat=int(p.heat), single_sqrt=True,
-                        pair_pass=int(self.choice[0]), pair_strip=int(self.choice[1]))
+                        pair_pass=int(self.choice[0]), pair_strip=int(self.choice[1]),
+                        cell_tile=int(self.choice[2]))
         api.actions_setup_corrector_constants(p.corr)
         api.actions_setup_physical_constants(p.phys)
         api.actions_setup_johnson_cook_constants(p.joco)
@@ -204,14 +206,12 @@ def serve(requests: list, hbm_budget: float = 250e9, steps_per_round: int = 5, o
     `max_neighbours`: neighbour-list capacity of the instances (0: the library default).
     `mode`: "batched" or "streams" (see the module notes); `use_graph`: batched rounds replay a
     captured step; `cell_tile`: cells per wavefront of the batches' neighbour pass (0 automatic).
-    `streams`: size of the stream pool the instances are spread over.  `tuning`
-    ("streams" only): an autotune.Table whose per-bucket pair-pass choice each instance runs with
-    (None: the library defaults).  Returns per-request records and p50 / p99 time to result and
-    the aggregate particle-steps/s."""
+    `streams`: size of the stream pool the instances are spread over.  `tuning`: an
+    autotune.Table; "streams": each instance runs with its bucket's (pair pass, strip, cell tile),
+    "batched": each precision's batch with its cell tile (None: the library defaults).  Returns
+    per-request records and p50 / p99 time to result and the aggregate particle-steps/s."""
     if mode not in ("batched", "streams"):
         raise ValueError(f"mode {mode!r}")
-    if mode == "batched" and tuning is not None:
-        raise ValueError("a tuning table applies to the streams mode (batches use one pair-pass form)")
     _lib.init()
     api.stream_pool(streams)
     load = [0] * streams  # active instances per stream
@@ -234,7 +234,8 @@ def serve(requests: list, hbm_budget: float = 250e9, steps_per_round: int = 5, o
             if mode == "batched":
                 grp = groups.get(r.precision)
                 if grp is None:
-                    grp = groups[r.precision] = _Group(len(groups) % streams, cell_tile)
+                    tile = tuning.batch_tile.get(r.precision, cell_tile) if tuning is not None else cell_tile
+                    grp = groups[r.precision] = _Group(len(groups) % streams, tile)
                 inst = Instance(r, grp.stream, max_neighbours=max_neighbours)
                 grp.batch.add(inst.particles, inst.grid, inst.tools, inst.blanking, None, dt=inst.dt,
                               t_current=inst.t)
@@ -243,7 +244,8 @@ def serve(requests: list, hbm_budget: float = 250e9, steps_per_round: int = 5, o
                 k = min(range(streams), key=lambda q: load[q])
                 load[k] += 1
                 nx, ny, nz = r.lattice()
-                choice = tuning.get(nx * ny * nz, r.precision) if tuning is not None else (api.MFG_PAIR_LIST, 0)
+                choice = (tuning.get(nx * ny * nz, r.precision) if tuning is not None
+                          else (api.MFG_PAIR_LIST, 0, cell_tile))
                 inst = Instance(r, k, choice, max_neighbours=max_neighbours)
             used += need
             setup_s += clock() - ts

@@ -169,9 +169,10 @@ def test_served_request_reports_device_errors(hip_device):
 
 
 def test_autotune_picks_the_fastest_per_bucket(tmp_path):
-    """Per (size bucket, precision) the first request seen is measured with every candidate and
-    the fastest wins (ties: the earlier candidate); instances of untuned sizes take the nearest
-    bucket's choice, untuned precisions the default; the table survives a JSON round trip."""
+    """Per (size bucket, precision) the first request seen is measured with every form / order
+    candidate, then with every cell tile under the fastest of those; the fastest wins (ties: the
+    earlier candidate); instances of untuned sizes take the nearest bucket's choice, untuned
+    precisions the default; the table survives a JSON round trip."""
     from iwf_mfree_gpu_3d_amd import autotune
     reqs = serve.sweep_requests(count=40, batched=True, n_min=5e3, n_max=2e5, steps=(10, 10), seed=4,
                                 fp32_share=1.0)
@@ -180,21 +181,27 @@ def test_autotune_picks_the_fastest_per_bucket(tmp_path):
     def fake(req, cand, steps, warmup):
         nx, ny, nz = req.lattice()
         seen.append((autotune.bucket(nx * ny * nz), cand))
-        # small instances prefer the sorted order, large ones strips of 16
-        fast = (api.MFG_PAIR_LIST, -1) if nx * ny * nz < 5e4 else (api.MFG_PAIR_LIST, 16)
-        return 1.0 if cand == fast else 2.0
+        # small instances prefer the sorted order and a tile of 4, large ones strips of 16 and 8
+        small = nx * ny * nz < 5e4
+        fast = (api.MFG_PAIR_LIST, -1) if small else (api.MFG_PAIR_LIST, 16)
+        t = 2.0 if cand[:2] != fast else 1.0
+        if cand[2] == (4 if small else 8):
+            t -= 0.5
+        return t
 
     table = autotune.tune(reqs, measure=fake)
     buckets = sorted({autotune.bucket(np.prod(r.lattice())) for r in reqs})
-    # FP32: the visiting orders only (the tiled form is not bit-identical there)
-    assert len(seen) == len(buckets) * 4 and all(c[0] == api.MFG_PAIR_LIST for _, c in seen)
+    # FP32: the visiting orders only (the tiled form is not bit-identical there), then the tiles
+    assert len(seen) == len(buckets) * (4 + len(autotune.TILES))
+    assert all(c[0] == api.MFG_PAIR_LIST for _, c in seen)
     assert len(autotune.candidates_for(_lib.MFG_FP64)) == len(autotune.CANDIDATES)
     for b in buckets:
-        want = (api.MFG_PAIR_LIST, -1) if 2 ** (b + 1) <= 5e4 else None
         got = table.choice[(b, _lib.MFG_FP32_KAHAN)]
-        if want:
-            assert got == want
-        assert got in autotune.CANDIDATES
+        if 2 ** (b + 1) <= 5e4:
+            assert got == (api.MFG_PAIR_LIST, -1, 4)
+        if 2 ** b >= 5e4:
+            assert got == (api.MFG_PAIR_LIST, 16, 8)
+        assert got[:2] in autotune.CANDIDATES and got[2] in autotune.TILES
     assert table.get(10 ** 9, _lib.MFG_FP32_KAHAN) == table.choice[(max(buckets), _lib.MFG_FP32_KAHAN)]
     assert table.get(10, _lib.MFG_FP32_KAHAN) == table.choice[(min(buckets), _lib.MFG_FP32_KAHAN)]
     assert table.get(10 ** 5, _lib.MFG_FP64) == autotune.DEFAULT
@@ -204,20 +211,43 @@ def test_autotune_picks_the_fastest_per_bucket(tmp_path):
     assert back.choice == table.choice and back.times == table.times
 
 
+def test_autotune_batch_tile_per_precision(tmp_path):
+    """The batch tile of each precision: its first `sample` requests measured as one batch with
+    every tile, the fastest kept (ties: the smaller tile); the serving loop hands it to that
+    precision's batch."""
+    from iwf_mfree_gpu_3d_amd import autotune
+    reqs = serve.sweep_requests(count=30, batched=True, n_min=5e3, n_max=2e5, steps=(10, 10), seed=4,
+                                fp32_share=0.5)
+    calls = []
+
+    def fake(batch_reqs, tile, steps, warmup):
+        calls.append((batch_reqs[0].precision, len(batch_reqs), tile))
+        want = 16 if batch_reqs[0].precision == _lib.MFG_FP32_KAHAN else 2
+        return 1.0 if tile == want else 3.0 - tile / 16.0
+
+    table = autotune.tune_batched(reqs, sample=8, measure=fake)
+    assert table.batch_tile == {_lib.MFG_FP32_KAHAN: 16, _lib.MFG_FP64: 2}
+    assert all(n == 8 for _, n, _ in calls) and len(calls) == 2 * len(autotune.TILES)
+    path = str(tmp_path / "tune.json")
+    table.save(path)
+    assert autotune.Table.load(path).batch_tile == table.batch_tile
+
+
 @pytest.mark.gpu
 def test_autotuned_batch_same_results(hip_device):
-    """A tuned table (measured on the GPU) changes only the pair-pass form (FP64) and the visiting
-    order, so a request's fields are bit-identical with and without it."""
+    """A tuned table (measured on the GPU) changes only the pair-pass form (FP64), the visiting
+    order and the cell tile, so a request's fields are bit-identical with and without it, on a
+    stream pool and batched."""
     from iwf_mfree_gpu_3d_amd import autotune
     reqs = serve.sweep_requests(count=2, batched=True, n_min=4e3, n_max=1.2e4, steps=(8, 8), seed=12,
                                 fp32_share=1.0)
     reqs[1].precision = _lib.MFG_FP64
     table = autotune.tune(reqs, steps=2, warmup=1)
     for (b, p), c in table.choice.items():
-        assert c in autotune.candidates_for(p)
+        assert c[:2] in autotune.candidates_for(p) and c[2] in autotune.TILES
     # force choices other than the default so the comparison exercises other kernels / orders
     for (b, p) in table.choice:
-        table.choice[(b, p)] = (api.MFG_PAIR_TILED, 0) if p == _lib.MFG_FP64 else (api.MFG_PAIR_LIST, 16)
+        table.choice[(b, p)] = (api.MFG_PAIR_TILED, 0, 8) if p == _lib.MFG_FP64 else (api.MFG_PAIR_LIST, 16, 2)
 
     def grab(store):
         def on_done(inst):
@@ -225,9 +255,12 @@ def test_autotuned_batch_same_results(hip_device):
             store[inst.req.rid] = {f: inst.particles.download(f)[order] for f in FIELDS}
         return on_done
 
-    plain, tuned = {}, {}
+    table.batch_tile = {_lib.MFG_FP64: 16, _lib.MFG_FP32_KAHAN: 2}
+    plain, tuned, tuned_batch = {}, {}, {}
     serve.serve(reqs, on_done=grab(plain), steps_per_round=4, mode="streams")
     serve.serve(reqs, on_done=grab(tuned), steps_per_round=4, tuning=table, mode="streams")
+    serve.serve(reqs, on_done=grab(tuned_batch), steps_per_round=4, tuning=table)
     for rid in plain:
         for f in FIELDS:
             np.testing.assert_array_equal(tuned[rid][f], plain[rid][f], err_msg=f"request {rid} field {f}")
+            np.testing.assert_array_equal(tuned_batch[rid][f], plain[rid][f], err_msg=f"request {rid} field {f}")
