@@ -75,9 +75,12 @@ enum {
   MFG_F_RHO_T = 18,     /* 1 */
   MFG_F_S_T = 19,       /* 9 */
   MFG_F_T_T = 20,       /* 1 */
+  MFG_F_S_DER = 21,     /* 9  stress divergence terms of the last interactions_monaghan call */
+  MFG_F_V_DER = 22,     /* 9  velocity gradient of the last interactions_monaghan call */
   MFG_F_HASH = 23,      /* 1  cell hash (INT_MAX when blanked) */
   MFG_F_IDX = 24,       /* 1  sort permutation of the last sort */
   MFG_F_ID = 25,        /* 1  persistent particle id (creation index) */
+  MFG_F_N = 26,         /* 3  contact normal of the last mfg_tool_compute_contact_force call */
   MFG_F_COUNT = 27
 };
 
@@ -335,6 +338,44 @@ int mfg_batch_size(const mfg_batch *b);
 int mfg_batch_set_cell_tile(mfg_batch *b, int cell_tile);
 int mfg_batch_get_time(const mfg_batch *b, int slot, double *t);
 int mfg_batch_step(mfg_batch *b, int n_steps, int record_forces, int use_graph);
+
+/* ---- per-function entry points ------------------------------------------------------------
+ * The reference's actions, pair passes, one tool's contact and the leap-frog phases, each one
+ * launch over the particle set's current (sorted) state, so that its leap_frog::step
+ * (src/leap_frog.cu:121-195) can be composed call by call.  mfg_leap_frog_step does the same
+ * work fused.  The arrays the reference keeps and the fused step does not (pressure, artificial
+ * stress, velocity gradient, stress divergence terms, contact normal, the leap-frog start copies)
+ * are allocated on the first call; they are step-local and not permuted by mfg_grid_sort.
+ * Time steps are arguments (the reference reads global_time_dt). */
+int mfg_actions_material_eos(mfg_particles *p);                           /* actions_gpu.h:41 */
+int mfg_actions_corrector_artificial_stress(mfg_particles *p);            /* actions_gpu.h:44 */
+int mfg_actions_material_stress_rate_jaumann(mfg_particles *p);           /* actions_gpu.h:45 */
+int mfg_actions_material_fric_heat_gen(mfg_particles *p, const double vel[3], double dt); /* :48 */
+int mfg_actions_contmech_continuity(mfg_particles *p);                    /* actions_gpu.h:51 */
+int mfg_actions_contmech_momentum(mfg_particles *p);                      /* actions_gpu.h:52 */
+int mfg_actions_contmech_advection(mfg_particles *p);                     /* actions_gpu.h:53 */
+int mfg_actions_plasticity_johnson_cook(mfg_particles *p, double dt);     /* actions_gpu.h:61 */
+int mfg_actions_perform_boundary_conditions_thermal(mfg_particles *p);   /* actions_gpu.h:64 */
+int mfg_actions_perform_boundary_conditions(mfg_particles *p);            /* actions_gpu.h:65 */
+int mfg_actions_move_tool_particles(mfg_particles *p, const mfg_tool *t, double dt); /* :68 */
+int mfg_debug_invalidate(mfg_particles *p);                               /* actions_gpu.h:79 */
+/* interactions_monaghan / interactions_heat_pse (src/interactions_gpu.h:51-52) on the current
+ * positions, with the grid's cell tables of the current order (after mfg_grid_sort): v_der,
+ * S_der, pos_t (XSPH), vel_t and, with Brookshaw conduction, T_t; T_t of the PSE sum */
+int mfg_interactions_monaghan(mfg_particles *p, mfg_grid *g);
+int mfg_interactions_heat_pse(mfg_particles *p, mfg_grid *g);
+/* tool_3d_gpu::compute_contact_force (src/tool_3d_gpu.h:120) at time t_current (after
+ * mfg_tool_update): fc, ft and the normal of every particle in contact; with record, the tool's
+ * force is added to slot `slot` of the force record */
+int mfg_tool_compute_contact_force(mfg_tool *t, mfg_particles *p, int record,
+                                   mfg_tool_forces *forces, int slot, double dt,
+                                   double t_current);
+/* leap_frog's phases (private in the reference, src/leap_frog.cu:30-119): the start-of-step
+ * copies; x = x0 + f x_t for position, velocity, stress, density and temperature (predict
+ * f = dt / 2, correct f = dt); the reset of the contact forces and normals */
+int mfg_leap_frog_init(mfg_leap_frog *lf, mfg_particles *p);
+int mfg_leap_frog_advance(mfg_leap_frog *lf, mfg_particles *p, double f);
+int mfg_leap_frog_reset_contact_forces(mfg_leap_frog *lf, mfg_particles *p);
 
 /* debug_check_valid_full (src/actions_gpu.cu:739-745): MFG_ERR_NONFINITE if an unblanked
  * particle holds a non-finite value */

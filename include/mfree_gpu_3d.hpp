@@ -230,6 +230,9 @@ class tool_3d_gpu {
     return mu;
   }
   bool is_alive(double t_current) const { return mfg_tool_is_alive(handle, t_current) == 1; }
+  // update_tool / compute_contact_force (src/tool_3d_gpu.h:82,120), at global_time_current
+  inline void update_tool();
+  inline void compute_contact_force(particle_gpu *p, bool record_force = false);
 };
 
 // ---- tool_forces (src/tool_3d_gpu.h:52-68) ------------------------------------------------------
@@ -318,6 +321,66 @@ double global_time_dt = 0., global_time_final = 0., global_time_current = 0.;
 int global_time_step = 0;
 #endif
 
+inline void tool_3d_gpu::update_tool() {
+  mfg_shim::check(mfg_tool_update(handle, global_time_current, global_time_dt));
+}
+inline void tool_3d_gpu::compute_contact_force(particle_gpu *p, bool record_force) {
+  int slot = 0;
+  for (size_t k = 0; k < global_tool.size(); k++)
+    if (global_tool[k] == this) slot = (int)k;
+  const bool rec = record_force && global_tool_forces;
+  mfg_shim::check(mfg_tool_compute_contact_force(handle, p->handle, rec ? 1 : 0,
+                                                 rec ? global_tool_forces->handle : nullptr, slot,
+                                                 global_time_dt, global_time_current));
+}
+
+// ---- actions and pair passes, one call each (src/actions_gpu.h:41-80,
+// src/interactions_gpu.h:51-52); time steps from the globals, as the reference reads them.  The
+// pair passes take the grid whose cell tables describe the current order (after g->sort).
+inline void material_eos(particle_gpu *p) { mfg_shim::check(mfg_actions_material_eos(p->handle)); }
+inline void corrector_artificial_stress(particle_gpu *p) {
+  mfg_shim::check(mfg_actions_corrector_artificial_stress(p->handle));
+}
+inline void material_stress_rate_jaumann(particle_gpu *p) {
+  mfg_shim::check(mfg_actions_material_stress_rate_jaumann(p->handle));
+}
+inline void material_fric_heat_gen(particle_gpu *p, mfg_shim::vec3 vel) {
+  const double v[3] = {vel.x, vel.y, vel.z};
+  mfg_shim::check(mfg_actions_material_fric_heat_gen(p->handle, v, global_time_dt));
+}
+inline void contmech_continuity(particle_gpu *p) {
+  mfg_shim::check(mfg_actions_contmech_continuity(p->handle));
+}
+inline void contmech_momentum(particle_gpu *p) {
+  mfg_shim::check(mfg_actions_contmech_momentum(p->handle));
+}
+inline void contmech_advection(particle_gpu *p) {
+  mfg_shim::check(mfg_actions_contmech_advection(p->handle));
+}
+inline void plasticity_johnson_cook(particle_gpu *p) {
+  mfg_shim::check(mfg_actions_plasticity_johnson_cook(p->handle, global_time_dt));
+}
+inline void perform_boundary_conditions_thermal(particle_gpu *p) {
+  mfg_shim::check(mfg_actions_perform_boundary_conditions_thermal(p->handle));
+}
+inline void perform_boundary_conditions(particle_gpu *p) {
+  mfg_shim::check(mfg_actions_perform_boundary_conditions(p->handle));
+}
+inline void actions_move_tool_particles(particle_gpu *p, tool_3d_gpu *t) {
+  mfg_shim::check(mfg_actions_move_tool_particles(p->handle, t->handle, global_time_dt));
+}
+inline void debug_invalidate(particle_gpu *p) { mfg_shim::check(mfg_debug_invalidate(p->handle)); }
+inline void perform_blanking(particle_gpu *p, blanking *b) {  // actions_gpu.h:57
+  const mfg_blanking_state bs = b->state();
+  mfg_shim::check(mfg_perform_blanking(p->handle, &bs));
+}
+inline void interactions_monaghan(particle_gpu *p, grid_base *g) {
+  mfg_shim::check(mfg_interactions_monaghan(p->handle, g->handle));
+}
+inline void interactions_heat_pse(particle_gpu *p, grid_base *g) {
+  mfg_shim::check(mfg_interactions_heat_pse(p->handle, g->handle));
+}
+
 // ---- leap_frog (src/leap_frog.h:37-50, src/leap_frog.cu:121-206) -------------------------------
 class leap_frog {
  public:
@@ -344,6 +407,45 @@ class leap_frog {
                                        tools.data(), (int)tools.size(), pb,
                                        global_tool_forces ? global_tool_forces->handle : nullptr,
                                        global_time_dt, global_time_current));
+    p->sync_counts();
+  }
+
+  // the same step composed call by call from the per-function entry points, in the order of
+  // src/leap_frog.cu:121-195 (the invalidation always on); same results, one launch per call
+  void step_by_actions(particle_gpu *p, grid_base *g, bool record_forces) {
+    if (global_blanking) {
+      global_blanking->update(global_time_current);
+      perform_blanking(p, global_blanking);
+    }
+    g->update_geometry(p, 2.0, global_time_current);
+    g->assign_hashes(p);
+    if (global_blanking) g->adapt_particle_number(p);
+    g->sort(p);
+    mfg_shim::check(mfg_leap_frog_init(handle, p->handle));
+    mfg_shim::check(mfg_leap_frog_advance(handle, p->handle, 0.5 * global_time_dt));
+    material_eos(p);
+    corrector_artificial_stress(p);
+    interactions_monaghan(p, g);
+    mfg_options opt;
+    mfg_shim::check(mfg_get_options(&opt));
+    if (opt.heat == MFG_HEAT_PSE) interactions_heat_pse(p, g);
+    material_stress_rate_jaumann(p);
+    contmech_continuity(p);
+    contmech_momentum(p);
+    contmech_advection(p);
+    debug_invalidate(p);
+    mfg_shim::check(mfg_leap_frog_advance(handle, p->handle, global_time_dt));
+    plasticity_johnson_cook(p);
+    mfg_shim::check(mfg_leap_frog_reset_contact_forces(handle, p->handle));
+    if (record_forces && global_tool_forces) global_tool_forces->reset();
+    for (tool_3d_gpu *t : global_tool) {
+      t->update_tool();
+      t->compute_contact_force(p, record_forces);
+    }
+    if (!global_tool.empty()) material_fric_heat_gen(p, global_tool[0]->get_vel());
+    perform_boundary_conditions(p);
+    perform_boundary_conditions_thermal(p);
+    if (!global_tool.empty()) actions_move_tool_particles(p, global_tool[0]);
     p->sync_counts();
   }
 };

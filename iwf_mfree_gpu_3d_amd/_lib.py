@@ -24,7 +24,8 @@ MFG_TOOL_WEDGE, MFG_TOOL_GRAIN = 0, 1
 F_POS, F_VEL, F_VEL_BC, F_FC, F_FT, F_H, F_RHO, F_P, F_S, F_R = range(10)
 F_EPS_PL, F_EPS_PL_DOT, F_T, F_FIXED, F_BLANKED, F_TOOL = range(10, 16)
 F_POS_T, F_VEL_T, F_RHO_T, F_S_T, F_T_T = range(16, 21)
-F_HASH, F_IDX, F_ID = 23, 24, 25
+F_S_DER, F_V_DER = 21, 22
+F_HASH, F_IDX, F_ID, F_N = 23, 24, 25, 26
 
 
 class MfgError(RuntimeError):
@@ -188,6 +189,24 @@ SIGNATURES = {
     "mfg_batch_get_time": (_i, [_vp, _i, _dp]),
     "mfg_batch_step": (_i, [_vp, _i, _i, _i]),
     "mfg_debug_check_valid": (_i, [_vp]),
+    "mfg_actions_material_eos": (_i, [_vp]),
+    "mfg_actions_corrector_artificial_stress": (_i, [_vp]),
+    "mfg_actions_material_stress_rate_jaumann": (_i, [_vp]),
+    "mfg_actions_material_fric_heat_gen": (_i, [_vp, _dp, _d]),
+    "mfg_actions_contmech_continuity": (_i, [_vp]),
+    "mfg_actions_contmech_momentum": (_i, [_vp]),
+    "mfg_actions_contmech_advection": (_i, [_vp]),
+    "mfg_actions_plasticity_johnson_cook": (_i, [_vp, _d]),
+    "mfg_actions_perform_boundary_conditions_thermal": (_i, [_vp]),
+    "mfg_actions_perform_boundary_conditions": (_i, [_vp]),
+    "mfg_actions_move_tool_particles": (_i, [_vp, _vp, _d]),
+    "mfg_debug_invalidate": (_i, [_vp]),
+    "mfg_interactions_monaghan": (_i, [_vp, _vp]),
+    "mfg_interactions_heat_pse": (_i, [_vp, _vp]),
+    "mfg_tool_compute_contact_force": (_i, [_vp, _vp, _i, _vp, _i, _d, _d]),
+    "mfg_leap_frog_init": (_i, [_vp, _vp]),
+    "mfg_leap_frog_advance": (_i, [_vp, _vp, _d]),
+    "mfg_leap_frog_reset_contact_forces": (_i, [_vp, _vp]),
     "mfg_profile_enable": (_i, [_i]),
     "mfg_profile_reset": (_i, []),
     "mfg_profile_count": (_i, []),

@@ -354,6 +354,17 @@ class tool_3d_gpu:
     def update_tool(self):
         check(_L().mfg_tool_update(self._h, global_time_current, global_time_dt))
 
+    def compute_contact_force(self, particles: particle_gpu, record_force: bool = False) -> None:
+        """tool_3d_gpu::compute_contact_force (src/tool_3d_gpu.h:120): fc, ft and the contact
+        normal of every particle in contact; with record_force the tool's force goes into its
+        slot (its index in global_tool) of global_tool_forces."""
+        forces, slot = None, 0
+        if record_force and global_tool_forces is not None:
+            forces = global_tool_forces._h
+            slot = next(k for k, t in enumerate(global_tool) if t is self)
+        check(_L().mfg_tool_compute_contact_force(self._h, particles._h, int(forces is not None), forces, slot,
+                                                  global_time_dt, global_time_current))
+
     def set_vel(self, v):
         a = _arr(v)
         check(_L().mfg_tool_set_vel(self._h, _dptr(a)))
@@ -493,6 +504,67 @@ def debug_check_valid_full(particles: particle_gpu) -> None:
     check(_L().mfg_debug_check_valid(particles._h))
 
 
+# ---- the reference's actions and pair passes, one call each (src/actions_gpu.h:41-80,
+# src/interactions_gpu.h:51-52); time steps from the process globals, as the reference reads them
+def material_eos(particles: particle_gpu) -> None:
+    check(_L().mfg_actions_material_eos(particles._h))
+
+
+def corrector_artificial_stress(particles: particle_gpu) -> None:
+    check(_L().mfg_actions_corrector_artificial_stress(particles._h))
+
+
+def material_stress_rate_jaumann(particles: particle_gpu) -> None:
+    check(_L().mfg_actions_material_stress_rate_jaumann(particles._h))
+
+
+def material_fric_heat_gen(particles: particle_gpu, vel) -> None:
+    v = np.ascontiguousarray(vel, dtype=np.float64).reshape(3)
+    check(_L().mfg_actions_material_fric_heat_gen(particles._h, _dptr(v), global_time_dt))
+
+
+def contmech_continuity(particles: particle_gpu) -> None:
+    check(_L().mfg_actions_contmech_continuity(particles._h))
+
+
+def contmech_momentum(particles: particle_gpu) -> None:
+    check(_L().mfg_actions_contmech_momentum(particles._h))
+
+
+def contmech_advection(particles: particle_gpu) -> None:
+    check(_L().mfg_actions_contmech_advection(particles._h))
+
+
+def plasticity_johnson_cook(particles: particle_gpu) -> None:
+    check(_L().mfg_actions_plasticity_johnson_cook(particles._h, global_time_dt))
+
+
+def perform_boundary_conditions_thermal(particles: particle_gpu) -> None:
+    check(_L().mfg_actions_perform_boundary_conditions_thermal(particles._h))
+
+
+def perform_boundary_conditions(particles: particle_gpu) -> None:
+    check(_L().mfg_actions_perform_boundary_conditions(particles._h))
+
+
+def actions_move_tool_particles(particles: particle_gpu, tool: "tool_3d_gpu") -> None:
+    check(_L().mfg_actions_move_tool_particles(particles._h, tool._h, global_time_dt))
+
+
+def debug_invalidate(particles: particle_gpu) -> None:
+    check(_L().mfg_debug_invalidate(particles._h))
+
+
+def interactions_monaghan(particles: particle_gpu, g: "grid_base") -> None:
+    """The pair pass on the current positions with g's cell tables of the current order (the
+    reference takes g's cell_start / cell_end)."""
+    check(_L().mfg_interactions_monaghan(particles._h, g._h))
+
+
+def interactions_heat_pse(particles: particle_gpu, g: "grid_base") -> None:
+    check(_L().mfg_interactions_heat_pse(particles._h, g._h))
+
+
 # ---- leap frog (src/leap_frog.h:37-50) --------------------------------------------------------
 class leap_frog:
     def __init__(self, num_part: int, num_cell: int):
@@ -506,6 +578,60 @@ class leap_frog:
         if h and _L is not None:  # at interpreter exit the module may already be torn down
             _L().mfg_leap_frog_destroy(h)
             self._h = None
+
+    # leap_frog's phases (private in the reference, src/leap_frog.cu:30-119), for stepping call by call
+    def init(self, particles: particle_gpu) -> None:
+        check(_L().mfg_leap_frog_init(self._h, particles._h))
+
+    def predict(self, particles: particle_gpu) -> None:
+        check(_L().mfg_leap_frog_advance(self._h, particles._h, 0.5 * global_time_dt))
+
+    def correct(self, particles: particle_gpu) -> None:
+        check(_L().mfg_leap_frog_advance(self._h, particles._h, global_time_dt))
+
+    def reset_contact_forces(self, particles: particle_gpu) -> None:
+        check(_L().mfg_leap_frog_reset_contact_forces(self._h, particles._h))
+
+    def step_by_actions(self, particles: particle_gpu, g: grid_base, record_forces: bool = False) -> None:
+        """leap_frog::step (src/leap_frog.cu:121-195) composed call by call from the per-function
+        entry points, in the reference's order (the invalidation always on, as in step).  The same
+        results as step(), one launch per call."""
+        if global_blanking is not None:
+            global_blanking.update()
+            perform_blanking(particles, global_blanking)
+        g.update_geometry(particles)
+        g.assign_hashes(particles)
+        if global_blanking is not None:
+            g.adapt_particle_number(particles)
+        g.sort(particles)
+        self.init(particles)
+        self.predict(particles)
+        material_eos(particles)
+        corrector_artificial_stress(particles)
+        interactions_monaghan(particles, g)
+        opts = _lib.options()
+        check(_L().mfg_get_options(C.byref(opts)))
+        if opts.heat == MFG_HEAT_PSE:
+            interactions_heat_pse(particles, g)
+        material_stress_rate_jaumann(particles)
+        contmech_continuity(particles)
+        contmech_momentum(particles)
+        contmech_advection(particles)
+        debug_invalidate(particles)
+        self.correct(particles)
+        plasticity_johnson_cook(particles)
+        self.reset_contact_forces(particles)
+        if record_forces and global_tool_forces is not None:
+            global_tool_forces.reset()
+        for tool in global_tool:
+            tool.update_tool()
+            tool.compute_contact_force(particles, record_forces)
+        if global_tool:
+            material_fric_heat_gen(particles, global_tool[0].get_vel())
+        perform_boundary_conditions(particles)
+        perform_boundary_conditions_thermal(particles)
+        if global_tool:
+            actions_move_tool_particles(particles, global_tool[0])
 
     def step(self, particles: particle_gpu, g: grid_base, record_forces: bool = False,
              allocation_free: bool = False) -> None:

@@ -19,7 +19,7 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 LIB = os.path.join(HERE, "libmfree_hip.so")
 BUILD = os.path.join(HERE, "_build")
 
-SOURCES = ["abi.hip", "k_grid.hip", "k_pair.hip", "k_tool.hip", "scene.cpp", "mesh_io.cpp"]
+SOURCES = ["abi.hip", "k_grid.hip", "k_pair.hip", "k_tool.hip", "k_actions.hip", "scene.cpp", "mesh_io.cpp"]
 ARCH = os.environ.get("MFG_OFFLOAD_ARCH", "gfx950")
 
 

@@ -205,7 +205,10 @@ struct mfg_particles {
   Counters *cnt_h = nullptr;   // pinned host mirror
   float *partials = nullptr;   // contact-force partial sums
   int contact_blocks = 0;
-  double *p_dbg = nullptr, *R_dbg = nullptr;
+  double *p_dbg = nullptr, *R_dbg = nullptr;  // pressure / artificial stress (per-action p and R)
+  // per-action entry points: velocity gradient, stress divergence terms, contact normal and the
+  // leap-frog start-of-step copies (lazily allocated)
+  double *vder = nullptr, *sder = nullptr, *nrm = nullptr, *x0 = nullptr;
   int *flag_d = nullptr;
   unsigned long long *stat_part = nullptr;  // per-workgroup (candidates, pairs) of the last list build
   int stat_cap = 0, stat_n = 0;
@@ -299,7 +302,8 @@ size_t padded(size_t n) { return (n + 63) / 64 * 64; }
 void free_particles(mfg_particles *p) {
   void *ptrs[] = {p->arena[0], p->arena[1], p->uarena[0], p->uarena[1], p->rec, p->nl4, p->pse2,
                   p->cs4, p->sums, p->trav, p->seg_pos, p->poison, p->nbr, p->count, p->key, p->key_sorted, p->order, p->blk, p->cnt, p->partials,
-                  p->p_dbg, p->R_dbg, p->flag_d, p->stat_part, p->cs_list, p->cs_n,
+                  p->p_dbg, p->R_dbg, p->vder, p->sder, p->nrm, p->x0, p->flag_d, p->stat_part,
+                  p->cs_list, p->cs_n,
                   p->cs_slot, p->cs_res};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
@@ -412,6 +416,42 @@ void permute(mfg_grid *g, mfg_particles *p, const Consts &c, int predict) {
   launch_permute_predict_ex(g_lib.stream, p->buf[p->cur], p->buf[nxt], p->N_init, gb, p->cnt, c,
                             predict, p->rec, p->nl4, p->pse2, p->cs4, p->p_dbg, p->R_dbg);
   p->cur = nxt;
+}
+
+// the per-action arrays of a particle set (allocated and zeroed on first use)
+int ensure_aux(mfg_particles *p) {
+  if (p->vder) return MFG_OK;
+  const size_t n = padded(p->N_init);
+  hipStream_t s = g_lib.stream;
+  if (!p->p_dbg) {
+    if (hipMalloc(&p->p_dbg, sizeof(double) * n) != hipSuccess ||
+        hipMalloc(&p->R_dbg, sizeof(double) * 6 * n) != hipSuccess)
+      return fail(MFG_ERR_NOMEM, "pressure / artificial stress arrays");
+    HIP_TRY(hipMemsetAsync(p->p_dbg, 0, sizeof(double) * n, s));
+    HIP_TRY(hipMemsetAsync(p->R_dbg, 0, sizeof(double) * 6 * n, s));
+  }
+  if (hipMalloc(&p->vder, sizeof(double) * 9 * n) != hipSuccess ||
+      hipMalloc(&p->sder, sizeof(double) * 9 * n) != hipSuccess ||
+      hipMalloc(&p->nrm, sizeof(double) * 3 * n) != hipSuccess ||
+      hipMalloc(&p->x0, sizeof(double) * 14 * n) != hipSuccess)
+    return fail(MFG_ERR_NOMEM, "per-action arrays (%u particles)", p->N_init);
+  HIP_TRY(hipMemsetAsync(p->vder, 0, sizeof(double) * 9 * n, s));
+  HIP_TRY(hipMemsetAsync(p->sder, 0, sizeof(double) * 9 * n, s));
+  HIP_TRY(hipMemsetAsync(p->nrm, 0, sizeof(double) * 3 * n, s));
+  HIP_TRY(hipMemsetAsync(p->x0, 0, sizeof(double) * 14 * n, s));
+  return MFG_OK;
+}
+
+Aux aux_of(const mfg_particles *p) {
+  Aux a;
+  a.p = p->p_dbg;
+  a.R = p->R_dbg;
+  a.vder = p->vder;
+  a.sder = p->sder;
+  a.nrm = p->nrm;
+  a.x0 = p->x0;
+  a.n = padded(p->N_init);
+  return a;
 }
 
 // buffers of the lane-parallel exhaustive contact search for n_tools tools
@@ -781,8 +821,10 @@ int mfg_field_width(int f) {
     case MFG_F_POS: case MFG_F_VEL: case MFG_F_VEL_BC: case MFG_F_FC: case MFG_F_FT:
     case MFG_F_POS_T: case MFG_F_VEL_T:
       return 3;
-    case MFG_F_S: case MFG_F_R: case MFG_F_S_T:
+    case MFG_F_S: case MFG_F_R: case MFG_F_S_T: case MFG_F_S_DER: case MFG_F_V_DER:
       return 9;
+    case MFG_F_N:
+      return 3;
     case MFG_F_H: case MFG_F_RHO: case MFG_F_P: case MFG_F_EPS_PL: case MFG_F_EPS_PL_DOT:
     case MFG_F_T: case MFG_F_FIXED: case MFG_F_BLANKED: case MFG_F_TOOL: case MFG_F_RHO_T:
     case MFG_F_T_T: case MFG_F_HASH: case MFG_F_IDX: case MFG_F_ID:
@@ -1065,6 +1107,22 @@ int mfg_particle_gpu_download(mfg_particles *p, int f, double *host) {
     case MFG_F_ID: return put_u(st.id, 3);
     case MFG_F_HASH: return put_u(p->key_sorted, 3);
     case MFG_F_IDX: return put_u(p->order, 3);
+    case MFG_F_S_DER:
+    case MFG_F_V_DER: {
+      const double *m = f == MFG_F_V_DER ? p->vder : p->sder;
+      if (!m) return fail(MFG_ERR_STATE, "no velocity gradient / stress divergence yet (interactions_monaghan computes them)");
+      const size_t n = padded(N);
+      for (int k = 0; k < 9; k++) {
+        if (!get(m + k * n)) return fail(MFG_ERR_HIP, "download failed");
+        for (unsigned i = 0; i < N; i++) host[9 * (size_t)i + k] = a[i];
+      }
+      return MFG_OK;
+    }
+    case MFG_F_N: {
+      if (!p->nrm) return fail(MFG_ERR_STATE, "no contact normals yet (tool compute_contact_force writes them)");
+      const size_t n = padded(N);
+      return put3(p->nrm, p->nrm + n, p->nrm + 2 * n);
+    }
     case MFG_F_P:
     case MFG_F_R: {
       if (!p->p_dbg) {
@@ -1699,6 +1757,31 @@ struct ProfOff {
 
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
+// what the contact kernels read of a tool at time t_current (after mfg_tool_update); the
+// closest triangle is searched per particle (no search list)
+void fill_tool_view(const mfg_tool *t, double t_current, ToolView &v) {
+  v.algo = t->algo;
+  v.tris = t->algo == MFG_CONTACT_HASHED ? t->boxed : t->tris;
+  v.n_tris = t->algo == MFG_CONTACT_HASHED ? t->n_boxed : t->n_tris;
+  v.cells = t->cells;
+  v.cand_off = t->algo == MFG_CONTACT_EXHAUSTIVE ? t->cand_off : nullptr;
+  v.cand_idx = t->cand_idx;
+  v.slot = nullptr;
+  v.res = nullptr;
+  for (int a = 0; a < 3; a++) v.corg[a] = t->gbbmin[a] + t->vel[a] * t->mesh_tc;
+  v.nx = t->gnx; v.ny = t->gny; v.nz = t->gnz;
+  v.active = (t->n_boxed > 0 && t_current >= t->birth && t_current <= t->death) ? 1 : 0;
+  for (int a = 0; a < 3; a++) {
+    v.bbmin[a] = t->bbmin[a];
+    v.bbmax[a] = t->bbmax[a];
+    v.vel_const[a] = t->vel_const[a];
+  }
+  v.dx = t->gdx;
+  v.mu = t->mu;
+  v.alpha = t->alpha;
+  v.mass = t->mass;
+}
+
 int step_impl(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int record,
               mfg_tool *const *tools, int n_tools, const mfg_blanking_state *blanking,
               mfg_tool_forces *forces, double dt, double t_current, const StepWork &work) {
@@ -1823,27 +1906,7 @@ int step_impl(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int record,
     mfg_tool *t = tools[k];
     rc = mfg_tool_update(t, t_current, dt);
     if (rc != MFG_OK) return rc;
-    ToolView &v = ts.t[k];
-    v.algo = t->algo;
-    v.tris = t->algo == MFG_CONTACT_HASHED ? t->boxed : t->tris;
-    v.n_tris = t->algo == MFG_CONTACT_HASHED ? t->n_boxed : t->n_tris;
-    v.cells = t->cells;
-    v.cand_off = t->algo == MFG_CONTACT_EXHAUSTIVE ? t->cand_off : nullptr;
-    v.cand_idx = t->cand_idx;
-    v.slot = nullptr;
-    v.res = nullptr;
-    for (int a = 0; a < 3; a++) v.corg[a] = t->gbbmin[a] + t->vel[a] * t->mesh_tc;
-    v.nx = t->gnx; v.ny = t->gny; v.nz = t->gnz;
-    v.active = (t->n_boxed > 0 && t_current >= t->birth && t_current <= t->death) ? 1 : 0;
-    for (int a = 0; a < 3; a++) {
-      v.bbmin[a] = t->bbmin[a];
-      v.bbmax[a] = t->bbmax[a];
-      v.vel_const[a] = t->vel_const[a];
-    }
-    v.dx = t->gdx;
-    v.mu = t->mu;
-    v.alpha = t->alpha;
-    v.mass = t->mass;
+    fill_tool_view(t, t_current, ts.t[k]);
   }
   if (n_tools > 0)
     for (int a = 0; a < 3; a++) ts.fric_vel[a] = tools[0]->vel[a];
@@ -1873,6 +1936,135 @@ int mfg_leap_frog_step(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int rec
                        mfg_tool *const *tools, int n_tools, const mfg_blanking_state *blanking,
                        mfg_tool_forces *forces, double dt, double t_current) {
   return step_impl(lf, p, g, record, tools, n_tools, blanking, forces, dt, t_current, StepWork{});
+}
+
+// ---- per-function entry points: the reference's actions, pair passes, one tool's contact and
+// the leap-frog phases, each one launch over the particle set's current (sorted) state ----
+static int run_action(mfg_particles *p, int which, double dt, const double *vel) {
+  NEED_LIB();
+  if (!p) return fail(MFG_ERR_ARG, "null particles");
+  int rc = ensure_aux(p);
+  if (rc != MFG_OK) return rc;
+  Consts c = make_consts(dt, p->precision);
+  // the reference's host-side switches (src/actions_gpu.cu:594-722)
+  if (which == ACT_JOHNSON_COOK && !c.plastic) return MFG_OK;
+  if (which == ACT_BC_THERMAL && !c.a_thermal) return MFG_OK;
+  if (which == ACT_FRIC_HEAT && !c.a_fric) return MFG_OK;
+  if (which == ACT_ART_STRESS && c.a_stresseps == 0.) return MFG_OK;
+  ProfScope ps("action");
+  launch_action(g_lib.stream, which, p->st(), p->N_init, p->cnt, c, aux_of(p), vel);
+  HIP_TRY(hipGetLastError());
+  return MFG_OK;
+}
+
+int mfg_actions_material_eos(mfg_particles *p) { return run_action(p, ACT_EOS, 0., nullptr); }
+int mfg_actions_corrector_artificial_stress(mfg_particles *p) {
+  return run_action(p, ACT_ART_STRESS, 0., nullptr);
+}
+int mfg_actions_material_stress_rate_jaumann(mfg_particles *p) {
+  return run_action(p, ACT_JAUMANN, 0., nullptr);
+}
+int mfg_actions_contmech_continuity(mfg_particles *p) {
+  return run_action(p, ACT_CONTINUITY, 0., nullptr);
+}
+int mfg_actions_contmech_momentum(mfg_particles *p) {
+  return run_action(p, ACT_MOMENTUM, 0., nullptr);
+}
+int mfg_actions_contmech_advection(mfg_particles *p) {
+  return run_action(p, ACT_ADVECTION, 0., nullptr);
+}
+int mfg_actions_plasticity_johnson_cook(mfg_particles *p, double dt) {
+  if (!(dt > 0.)) return fail(MFG_ERR_ARG, "time step %g", dt);
+  return run_action(p, ACT_JOHNSON_COOK, dt, nullptr);
+}
+int mfg_actions_material_fric_heat_gen(mfg_particles *p, const double vel[3], double dt) {
+  if (!vel) return fail(MFG_ERR_ARG, "null tool velocity");
+  return run_action(p, ACT_FRIC_HEAT, dt, vel);
+}
+int mfg_actions_perform_boundary_conditions(mfg_particles *p) {
+  return run_action(p, ACT_BC, 0., nullptr);
+}
+int mfg_actions_perform_boundary_conditions_thermal(mfg_particles *p) {
+  return run_action(p, ACT_BC_THERMAL, 0., nullptr);
+}
+int mfg_actions_move_tool_particles(mfg_particles *p, const mfg_tool *t, double dt) {
+  if (!t) return fail(MFG_ERR_ARG, "null tool");
+  return run_action(p, ACT_MOVE_TOOL, dt, t->vel);
+}
+int mfg_debug_invalidate(mfg_particles *p) { return run_action(p, ACT_INVALIDATE, 0., nullptr); }
+
+int mfg_leap_frog_init(mfg_leap_frog *lf, mfg_particles *p) {
+  if (!lf) return fail(MFG_ERR_ARG, "null leap_frog");
+  return run_action(p, ACT_LF_INIT, 0., nullptr);
+}
+int mfg_leap_frog_advance(mfg_leap_frog *lf, mfg_particles *p, double f) {
+  NEED_LIB();
+  if (!lf || !p) return fail(MFG_ERR_ARG, "null argument");
+  int rc = ensure_aux(p);
+  if (rc != MFG_OK) return rc;
+  launch_lf_advance(g_lib.stream, p->st(), p->N_init, aux_of(p), f);
+  HIP_TRY(hipGetLastError());
+  return MFG_OK;
+}
+int mfg_leap_frog_reset_contact_forces(mfg_leap_frog *lf, mfg_particles *p) {
+  if (!lf) return fail(MFG_ERR_ARG, "null leap_frog");
+  return run_action(p, ACT_RESET_CONTACT, 0., nullptr);
+}
+
+// the pair records of the current state; the grid's cell tables must describe its order
+static int interaction_records(mfg_particles *p, mfg_grid *g, Consts &c) {
+  NEED_LIB();
+  if (!p || !g) return fail(MFG_ERR_ARG, "null argument");
+  int rc = ensure_aux(p);
+  if (rc != MFG_OK) return rc;
+  c = make_consts(0., p->precision);
+  launch_act_records(g_lib.stream, p->st(), p->N_init, p->cnt, c, aux_of(p), p->rec, p->nl4,
+                     p->pse2, p->cs4);
+  HIP_TRY(hipGetLastError());
+  return MFG_OK;
+}
+
+int mfg_interactions_monaghan(mfg_particles *p, mfg_grid *g) {
+  Consts c;
+  int rc = interaction_records(p, g, c);
+  if (rc != MFG_OK) return rc;
+  rc = run_neighbours(g, p, c, 0);
+  if (rc != MFG_OK) return rc;
+  const GridInfo gi = g->info();
+  launch_pair(g_lib.stream, p->N_init, p->st(), p->rec, p->nbr, p->nl_stride, p->count, c, p->cnt,
+              gi, g->count, p->key_sorted, 0, p->sums, p->sum_stride, nullptr, p->cs4);
+  launch_act_sums(g_lib.stream, p->st(), p->N_init, p->cnt, c, aux_of(p), p->sums, p->sum_stride,
+                  p->poison);
+  HIP_TRY(hipGetLastError());
+  return MFG_OK;
+}
+
+int mfg_interactions_heat_pse(mfg_particles *p, mfg_grid *g) {
+  Consts c;
+  int rc = interaction_records(p, g, c);
+  if (rc != MFG_OK) return rc;
+  if (!c.pse_on) return MFG_OK;  // no workpiece conduction constants (src/interactions_gpu.cu:536)
+  return run_neighbours(g, p, c, 1);
+}
+
+int mfg_tool_compute_contact_force(mfg_tool *t, mfg_particles *p, int record,
+                                   mfg_tool_forces *forces, int slot, double dt,
+                                   double t_current) {
+  NEED_LIB();
+  if (!t || !p) return fail(MFG_ERR_ARG, "null argument");
+  if (record && (!forces || slot < 0 || (unsigned)slot >= forces->n))
+    return fail(MFG_ERR_ARG, "recording needs a force record with a slot for this tool");
+  int rc = ensure_aux(p);
+  if (rc != MFG_OK) return rc;
+  ToolView v;
+  std::memset(&v, 0, sizeof(v));
+  fill_tool_view(t, t_current, v);
+  hipStream_t s = g_lib.stream;
+  launch_act_contact(s, p->st(), p->N_init, v, dt, p->cnt, aux_of(p), record ? 1 : 0, p->partials,
+                     p->contact_blocks);
+  if (record) launch_force_reduce(s, p->partials, p->contact_blocks, 1, forces->d + 3 * slot);
+  HIP_TRY(hipGetLastError());
+  return MFG_OK;
 }
 
 int mfg_leap_frog_step_noalloc(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int record,

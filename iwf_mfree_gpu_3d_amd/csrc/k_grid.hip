@@ -16,6 +16,7 @@
 // block-wise compaction.  Cell start/end of the reference are off[c], off[c+1] for non-empty
 // cells.
 #include "mfg_device.hpp"
+#include "mfg_records.hpp"
 
 namespace mfg {
 
@@ -246,65 +247,6 @@ __global__ void __launch_bounds__(BS) k_fixup(const int *off, long num_cell, int
   fixup(blockIdx.x, off, num_cell, order);
 }
 
-// what the CSPM sum reads of a neighbour, 32 bytes: FP64 (x, y, z, vol), the volume of a tool
-// particle replaced by the NaN pattern CSPM_TOOL_BITS (no volume m / rho has that payload, so a
-// workpiece particle of negative or non-finite density still counts as the reference counts it);
-// FP32 the record's first 32 bytes (x, y, z, {tool, vol})
-template <class RecT>
-__device__ __forceinline__ double4 cspm_entry(const double *xh, double vol, bool tool);
-template <>
-__device__ __forceinline__ double4 cspm_entry<Rec64>(const double *xh, double vol, bool tool) {
-  return make_double4(xh[0], xh[1], xh[2], tool ? __longlong_as_double(CSPM_TOOL_BITS) : vol);
-}
-template <>
-__device__ __forceinline__ double4 cspm_entry<Rec32>(const double *xh, double vol, bool tool) {
-  struct {
-    float tool, vol;
-  } tv = {tool ? 1.f : 0.f, (float)vol};
-  return make_double4(xh[0], xh[1], xh[2], __builtin_bit_cast(double, tv));
-}
-
-template <class RecT>
-__device__ __forceinline__ void store_rec(RecT *rec, unsigned i, const double *xh, double h,
-                                          const double *vh, double rho, const double *A,
-                                          const double *R, double T, double vol, double c,
-                                          double tool);
-
-template <>
-__device__ __forceinline__ void store_rec<Rec64>(Rec64 *rec, unsigned i, const double *xh,
-                                                 double h, const double *vh, double rho,
-                                                 const double *A, const double *R, double T,
-                                                 double vol, double c, double tool) {
-  Rec64 r;
-  r.x = xh[0]; r.y = xh[1]; r.z = xh[2]; r.h = h;
-  r.vx = vh[0]; r.vy = vh[1]; r.vz = vh[2]; r.rho = rho;
-#pragma unroll
-  for (int k = 0; k < 6; k++) {
-    r.A[k] = A[k];
-    r.R[k] = R[k];
-  }
-  r.T = T; r.vol = vol; r.c = c; r.tool = tool;
-  rec[i] = r;
-}
-
-template <>
-__device__ __forceinline__ void store_rec<Rec32>(Rec32 *rec, unsigned i, const double *xh,
-                                                 double h, const double *vh, double rho,
-                                                 const double *A, const double *R, double T,
-                                                 double vol, double c, double tool) {
-  Rec32 r;
-  r.x = xh[0]; r.y = xh[1]; r.z = xh[2];
-  r.h = (float)h; r.vx = (float)vh[0]; r.vy = (float)vh[1]; r.vz = (float)vh[2];
-  r.rho = (float)rho;
-#pragma unroll
-  for (int k = 0; k < 6; k++) {
-    r.A[k] = (float)A[k];
-    r.R[k] = (float)R[k];
-  }
-  r.T = (float)T; r.vol = (float)vol; r.c = (float)c; r.tool = (float)tool; r.pad = 0.f;
-  rec[i] = r;
-}
-
 // gather the persistent state by the sort permutation, then predict the half step and build the
 // pair records (EOS and artificial stress included)
 template <class RecT>
@@ -381,23 +323,7 @@ __device__ __forceinline__ void permute_predict(unsigned b, const State &src, co
     p = c0 * c0 * (rhoh - c.a_rho0);
     if (c.a_stresseps != 0.) art_stress(c.a_stresseps, rhoh, p, Sh, R);
   }
-  double rho21 = 1. / (rhoh * rhoh);
-  double A[6] = {(Sh[0] - p) * rho21, (Sh[1] - p) * rho21, (Sh[2] - p) * rho21,
-                 Sh[3] * rho21,       Sh[4] * rho21,       Sh[5] * rho21};
-  double vol = c.i_mass / rhoh;
-  double cs = sqrt_site(c.i_K / rhoh, c.single_sqrt);
-  store_rec<RecT>(rec, i, xh, h, vh, rhoh, A, R, Th, vol, cs, tool ? 1.0 : 0.0);
-  cs4[i] = cspm_entry<RecT>(xh, vol, tool);
-  // candidate flag of the neighbour pass: 1 blanked, 2 a non-finite value among the fields a
-  // neighbour contributes (the reference's loops visit every candidate of the stencil, so
-  // 0 * NaN carries such a value into the sums of every particle around it)
-  bool finite = isfinite(vol) && isfinite(Th);
-#pragma unroll
-  for (int k = 0; k < 3; k++) finite = finite && isfinite(xh[k]) && isfinite(vh[k]);
-#pragma unroll
-  for (int k = 0; k < 6; k++) finite = finite && isfinite(A[k]) && isfinite(R[k]);
-  nl4[i] = make_double4(xh[0], xh[1], xh[2], blanked ? 1.0 : (finite ? 0.0 : 2.0));
-  if (pse2) pse2[i] = make_double2(Th, c.i_mass / rhoh);  // (T, m / rho) of the PSE sum
+  emit_record<RecT>(i, xh, h, vh, rhoh, Sh, p, R, Th, tool, blanked, c, rec, nl4, pse2, cs4);
   if (p_dbg) {
     p_dbg[i] = p;
 #pragma unroll

@@ -676,46 +676,7 @@ __device__ __forceinline__ void particle_update(const State &st, unsigned i, uin
   double rho = rho0 + dt * nrho_t;
   double T = T0 + dt * nT_t;
   // Johnson-Cook radial return (src/actions_gpu.cu:371-449)
-  if (c.plastic && !tool) {
-    double sxx = S[0], syy = S[1], szz = S[2], sxy = S[3], syz = S[4], szx = S[5];
-    double nrm = sqrt_rn(sxx * sxx + syy * syy + szz * szz +
-                            2 * (sxy * sxy + syz * syz + szx * szx));
-    double ep = st.eps_pl[i], epd = st.eps_pl_dot[i];
-    double Tc = T;
-    if (c.jc_clamp != 0.) Tc = (Tc > c.jc_Tmelt) ? c.jc_Tmelt - 1 : Tc;
-    double svm = sqrt_rn(3.0 / 2.0) * nrm;
-    double sy = sigma_yield(c, ep, epd, Tc);
-    if (svm < sy) {
-      st.eps_pl_dot[i] = 0.;
-    } else {
-      bool failed = false;
-      double dl = solve_secant(c, fmax(epd * dt * sqrt_rn(2. / 3.), 1e-8), 1e-6, nrm, ep, Tc,
-                               dt, c.a_G, failed);
-      double ep_new, epd_new;
-      if (c.single_sqrt) {
-        float f = sqrt_rn((float)(2.0 / 3.0)) * fmaxf((float)dl, 0.f);
-        ep_new = ep + (double)f;
-        epd_new = (double)f / dt;
-      } else {
-        double f = sqrt_rn(2.0 / 3.0) * fmax(dl, 0.);
-        ep_new = ep + f;
-        epd_new = sqrt_rn(2.0 / 3.0) * fmax(dl, 0.) / dt;
-      }
-      st.eps_pl[i] = ep_new;
-      st.eps_pl_dot[i] = epd_new;
-      const double mu = c.a_G;
-      S[0] = sxx - sxx / nrm * dl * 2. * mu;
-      S[1] = syy - syy / nrm * dl * 2. * mu;
-      S[2] = szz - szz / nrm * dl * 2. * mu;
-      S[3] = sxy - sxy / nrm * dl * 2. * mu;
-      S[4] = syz - syz / nrm * dl * 2. * mu;
-      S[5] = szx - szx / nrm * dl * 2. * mu;
-      if (c.twp_tq != 0.) {
-        double sy2 = sigma_yield(c, ep_new, epd_new, Tc);
-        T += c.twp_tq / (c.twp_cp * rho) * (ep_new - ep) * sy2;
-      }
-    }
-  }
+  if (c.plastic && !tool) jc_radial_return(c, dt, st, i, S, T, rho);
   st.x[i] = x[0]; st.y[i] = x[1]; st.z[i] = x[2];
   st.vx[i] = v[0]; st.vy[i] = v[1]; st.vz[i] = v[2];
 #pragma unroll

@@ -512,6 +512,75 @@ __global__ void k_tool_update(Tri *tris, const double *p_init, int n, Tri *tris2
   move_tri(tris, p_init, k, t, v);
 }
 
+// Contact of the particle at qp (velocity vel, tangential force of before the contact ft) with
+// one tool (src/tool_3d_gpu.cu:455-583 exhaustive, :585-681 hashed): false outside the tool's
+// moving box, outside the mesh or for an inactive tool; else the normal penalty force fN, the
+// Coulomb-limited tangential force fT and the contact normal nrm.
+__device__ __forceinline__ bool tool_contact(const ToolView &tv, unsigned i, V3 qp, V3 vel, V3 ft,
+                                             double dt, V3 &fN, V3 &fT, V3 &nrm) {
+  if (!tv.active) return false;
+  if (!in_tool_box(tv, qp)) return false;
+  V3 cp = v3(0, 0, 0);
+  nrm = v3(0, 0, 0);
+  if (tv.algo == MFG_CONTACT_HASHED) {
+    int ix = (int)(unsigned)((qp.x - tv.bbmin[0]) / tv.dx);
+    int iy = (int)(unsigned)((qp.y - tv.bbmin[1]) / tv.dx);
+    int iz = (int)(unsigned)((qp.z - tv.bbmin[2]) / tv.dx);
+    // inside the (moving) box the cell index is < n; clamp against rounding at the edge
+    ix = min(ix, tv.nx - 1);
+    iy = min(iy, tv.ny - 1);
+    iz = min(iz, tv.nz - 1);
+    if (!contact_by_ray(tv.tris, tv.cells, tv.ny, tv.nz, ix, iy, iz, qp, tv.bbmin, tv.dx))
+      return false;
+    contact_by_normal(tv.tris, tv.cells, tv.nx, tv.ny, tv.nz, ix, iy, iz, qp, cp, nrm);
+  } else {
+    // cell of the point in the mesh's own frame; outside the grid, the full loop
+    const double fx = (qp.x - tv.corg[0]) / tv.dx, fy = (qp.y - tv.corg[1]) / tv.dx,
+                 fz = (qp.z - tv.corg[2]) / tv.dx;
+    const bool in_grid = tv.cand_off && fx >= 0. && fy >= 0. && fz >= 0. &&
+                         fx < tv.nx && fy < tv.ny && fz < tv.nz;
+    const int sl = tv.slot ? tv.slot[i] : -1;
+    if (sl >= 0) {
+      const double *r = tv.res + 6 * (size_t)sl;
+      cp = v3(r[0], r[1], r[2]);
+      nrm = v3(r[3], r[4], r[5]);
+    } else if (in_grid) {
+      const int cell = ((int)fx * tv.ny + (int)fy) * tv.nz + (int)fz;
+      closest_by_candidates(tv, qp, cell, cp, nrm);
+    } else {
+      double min_dist = FLT_MAX;
+      for (int q = 0; q < tv.n_tris; q++) {
+        V3 cc, n2;
+        closest_point(tv.tris[q], qp, cc, n2);
+        double d = dist2(cc, qp);
+        if (d < min_dist) {
+          min_dist = d;
+          cp = cc;
+          nrm = n2;
+        }
+      }
+    }
+    if (vdot(vsub(cp, qp), nrm) > 0.) return false;
+  }
+  const double dt2 = dt * dt;
+  double gN = vdot(vsub(qp, cp), nrm);
+  double s = -tv.mass * gN;
+  fN = v3(s * nrm.x / dt2 * tv.alpha, s * nrm.y / dt2 * tv.alpha, s * nrm.z / dt2 * tv.alpha);
+  fT = v3(0., 0., 0.);
+  if (tv.mu != 0.) {
+    V3 vv = vsub(vel, v3(tv.vel_const[0], tv.vel_const[1], tv.vel_const[2]));
+    V3 vr = v3(vv.x - vv.x * nrm.x, vv.y - vv.y * nrm.y, vv.z - vv.z * nrm.z);
+    double am = tv.alpha * tv.mass;
+    V3 kd = v3(am * vr.x / dt, am * vr.y / dt, am * vr.z / dt);
+    double fy = tv.mu * vlen(fN);
+    V3 fstar = vsub(ft, kd);
+    double lf = vlen(fstar);
+    if (lf > fy) fT = v3(fy * fstar.x / lf, fy * fstar.y / lf, fy * fstar.z / lf);
+    else fT = fstar;
+  }
+  return true;
+}
+
 // ---- contact, friction heat, boundary conditions, tool particle motion ------------------------
 __device__ __forceinline__ void contact_bc(unsigned b, const State &st, unsigned n_init,
                                            const ToolSet &ts, const Consts &c,
@@ -535,65 +604,8 @@ __device__ __forceinline__ void contact_bc(unsigned b, const State &st, unsigned
         const V3 qp = v3(st.x[i], st.y[i], st.z[i]);
         for (int k = 0; k < ts.n; k++) {
           const ToolView &tv = ts.t[k];
-          if (!tv.active) continue;
-          if (!in_tool_box(tv, qp)) continue;
-          V3 cp = v3(0, 0, 0), nrm = v3(0, 0, 0);
-          if (tv.algo == MFG_CONTACT_HASHED) {
-            int ix = (int)(unsigned)((qp.x - tv.bbmin[0]) / tv.dx);
-            int iy = (int)(unsigned)((qp.y - tv.bbmin[1]) / tv.dx);
-            int iz = (int)(unsigned)((qp.z - tv.bbmin[2]) / tv.dx);
-            // inside the (moving) box the cell index is < n; clamp against rounding at the edge
-            ix = min(ix, tv.nx - 1);
-            iy = min(iy, tv.ny - 1);
-            iz = min(iz, tv.nz - 1);
-            if (!contact_by_ray(tv.tris, tv.cells, tv.ny, tv.nz, ix, iy, iz, qp, tv.bbmin, tv.dx))
-              continue;
-            contact_by_normal(tv.tris, tv.cells, tv.nx, tv.ny, tv.nz, ix, iy, iz, qp, cp, nrm);
-          } else {
-            // cell of the point in the mesh's own frame; outside the grid, the full loop
-            const double fx = (qp.x - tv.corg[0]) / tv.dx, fy = (qp.y - tv.corg[1]) / tv.dx,
-                         fz = (qp.z - tv.corg[2]) / tv.dx;
-            const bool in_grid = tv.cand_off && fx >= 0. && fy >= 0. && fz >= 0. &&
-                                 fx < tv.nx && fy < tv.ny && fz < tv.nz;
-            const int sl = tv.slot ? tv.slot[i] : -1;
-            if (sl >= 0) {
-              const double *r = tv.res + 6 * (size_t)sl;
-              cp = v3(r[0], r[1], r[2]);
-              nrm = v3(r[3], r[4], r[5]);
-            } else if (in_grid) {
-              const int cell = ((int)fx * tv.ny + (int)fy) * tv.nz + (int)fz;
-              closest_by_candidates(tv, qp, cell, cp, nrm);
-            } else {
-              double min_dist = FLT_MAX;
-              for (int q = 0; q < tv.n_tris; q++) {
-                V3 cc, n2;
-                closest_point(tv.tris[q], qp, cc, n2);
-                double d = dist2(cc, qp);
-                if (d < min_dist) {
-                  min_dist = d;
-                  cp = cc;
-                  nrm = n2;
-                }
-              }
-            }
-            if (vdot(vsub(cp, qp), nrm) > 0.) continue;
-          }
-          const double dt = c.dt, dt2 = dt * dt;
-          double gN = vdot(vsub(qp, cp), nrm);
-          double s = -tv.mass * gN;
-          V3 fN = v3(s * nrm.x / dt2 * tv.alpha, s * nrm.y / dt2 * tv.alpha, s * nrm.z / dt2 * tv.alpha);
-          V3 fT = v3(0., 0., 0.);
-          if (tv.mu != 0.) {
-            V3 vv = vsub(vel, v3(tv.vel_const[0], tv.vel_const[1], tv.vel_const[2]));
-            V3 vr = v3(vv.x - vv.x * nrm.x, vv.y - vv.y * nrm.y, vv.z - vv.z * nrm.z);
-            double am = tv.alpha * tv.mass;
-            V3 kd = v3(am * vr.x / dt, am * vr.y / dt, am * vr.z / dt);
-            double fy = tv.mu * vlen(fN);
-            V3 fstar = vsub(ft, kd);
-            double lf = vlen(fstar);
-            if (lf > fy) fT = v3(fy * fstar.x / lf, fy * fstar.y / lf, fy * fstar.z / lf);
-            else fT = fstar;
-          }
+          V3 fN, fT, nrm;
+          if (!tool_contact(tv, i, qp, vel, ft, c.dt, fN, fT, nrm)) continue;
           fc = fN;
           ft = fT;
           nn = nrm;
@@ -828,6 +840,57 @@ void launch_contact_bc(hipStream_t s, const State &st, unsigned n_init, const To
 void launch_force_reduce(hipStream_t s, const float *partials, int nblocks, int n_tools,
                          float *forces) {
   hipLaunchKernelGGL(k_force_reduce, dim3(1), dim3(64), 0, s, partials, nblocks, n_tools, forces);
+}
+
+// one tool's contact (tool_3d_gpu::compute_contact_force, src/tool_3d_gpu.cu:455-681): fc, ft
+// and the normal of every particle in contact (the tangential force of before the call is the
+// friction law's previous force), and the force record's per-workgroup partials
+__global__ void __launch_bounds__(BST)
+    k_act_contact(State st, unsigned n_init, ToolView tv, double dt, const Counters *cnt, Aux a,
+                  int record, float *partials) {
+  __shared__ float red[3][BST / 64];
+  const unsigned i = blockIdx.x * BST + threadIdx.x;
+  float fr[3] = {0.f, 0.f, 0.f};
+  if (i < n_init && i < cnt->N) {
+    const uint32_t fl = st.flags[i];
+    if (!(fl & FL_BLANKED) && !(fl & FL_TOOL)) {
+      const V3 qp = v3(st.x[i], st.y[i], st.z[i]);
+      const V3 vel = v3(st.vx[i], st.vy[i], st.vz[i]);
+      const V3 ft = v3(st.ftx[i], st.fty[i], st.ftz[i]);
+      V3 fN, fT, nrm;
+      if (tool_contact(tv, i, qp, vel, ft, dt, fN, fT, nrm)) {
+        st.fcx[i] = fN.x; st.fcy[i] = fN.y; st.fcz[i] = fN.z;
+        st.ftx[i] = fT.x; st.fty[i] = fT.y; st.ftz[i] = fT.z;
+        a.nrm[i] = nrm.x; a.nrm[a.n + i] = nrm.y; a.nrm[2 * a.n + i] = nrm.z;
+        fr[0] = (float)(fN.x + fT.x);
+        fr[1] = (float)(fN.y + fT.y);
+        fr[2] = (float)(fN.z + fT.z);
+        if (tv.algo != MFG_CONTACT_HASHED)
+          for (int q = 0; q < 3; q++) fr[q] = isnan(fr[q]) ? 0.f : fr[q];
+      }
+    }
+  }
+  if (!record) return;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int q = 0; q < 3; q++) {
+    float v = fr[q];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0) red[q][wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 3; q++) {
+      float v = 0.f;
+      for (int w = 0; w < BST / 64; w++) v += red[q][w];
+      partials[(size_t)blockIdx.x * MAX_TOOLS * 3 + q] = v;
+    }
+}
+
+void launch_act_contact(hipStream_t s, const State &st, unsigned n_init, const ToolView &tv,
+                        double dt, const Counters *cnt, const Aux &a, int record, float *partials,
+                        int nblocks) {
+  hipLaunchKernelGGL(k_act_contact, dim3(nblocks), dim3(BST), 0, s, st, n_init, tv, dt, cnt, a,
+                     record, partials);
 }
 
 // clears n floats with a kernel (a memset node in a captured graph is what this avoids)

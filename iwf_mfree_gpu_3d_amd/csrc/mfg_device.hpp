@@ -344,6 +344,51 @@ __device__ __forceinline__ double solve_secant(const Consts &c, double init, dou
   }
 }
 
+// Johnson-Cook radial return of one particle (src/actions_gpu.cu:371-449,
+// src/plasticity.cuh:28-93): S (xx yy zz xy yz zx) after the step is returned to the yield surface
+// when its von Mises stress reaches the yield stress at (eps_pl, eps_pl_dot, T); eps_pl,
+// eps_pl_dot and, with plastic heating, T follow.  rho: the density after the step.
+__device__ __forceinline__ void jc_radial_return(const Consts &c, double dt, const State &st,
+                                                 unsigned i, double S[6], double &T, double rho) {
+  double sxx = S[0], syy = S[1], szz = S[2], sxy = S[3], syz = S[4], szx = S[5];
+  double nrm = sqrt_rn(sxx * sxx + syy * syy + szz * szz + 2 * (sxy * sxy + syz * syz + szx * szx));
+  double ep = st.eps_pl[i], epd = st.eps_pl_dot[i];
+  double Tc = T;
+  if (c.jc_clamp != 0.) Tc = (Tc > c.jc_Tmelt) ? c.jc_Tmelt - 1 : Tc;
+  double svm = sqrt_rn(3.0 / 2.0) * nrm;
+  double sy = sigma_yield(c, ep, epd, Tc);
+  if (svm < sy) {
+    st.eps_pl_dot[i] = 0.;
+    return;
+  }
+  bool failed = false;
+  double dl = solve_secant(c, fmax(epd * dt * sqrt_rn(2. / 3.), 1e-8), 1e-6, nrm, ep, Tc, dt, c.a_G,
+                           failed);
+  double ep_new, epd_new;
+  if (c.single_sqrt) {
+    float f = sqrt_rn((float)(2.0 / 3.0)) * fmaxf((float)dl, 0.f);
+    ep_new = ep + (double)f;
+    epd_new = (double)f / dt;
+  } else {
+    double f = sqrt_rn(2.0 / 3.0) * fmax(dl, 0.);
+    ep_new = ep + f;
+    epd_new = sqrt_rn(2.0 / 3.0) * fmax(dl, 0.) / dt;
+  }
+  st.eps_pl[i] = ep_new;
+  st.eps_pl_dot[i] = epd_new;
+  const double mu = c.a_G;
+  S[0] = sxx - sxx / nrm * dl * 2. * mu;
+  S[1] = syy - syy / nrm * dl * 2. * mu;
+  S[2] = szz - szz / nrm * dl * 2. * mu;
+  S[3] = sxy - sxy / nrm * dl * 2. * mu;
+  S[4] = syz - syz / nrm * dl * 2. * mu;
+  S[5] = szx - szx / nrm * dl * 2. * mu;
+  if (c.twp_tq != 0.) {
+    double sy2 = sigma_yield(c, ep_new, epd_new, Tc);
+    T += c.twp_tq / (c.twp_cp * rho) * (ep_new - ep) * sy2;
+  }
+}
+
 // ---- cubic spline (src/kernels.cuh:26-69), returns W and grad W ------------------------------
 __device__ __forceinline__ void cubic_spline64(double xij, double yij, double zij, double rr2,
                                                double h1, double fac, int single, double &w,

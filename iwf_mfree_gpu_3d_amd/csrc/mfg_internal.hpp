@@ -317,6 +317,41 @@ void launch_batch_permute(hipStream_t s, const BatchLaunch &L);
 void launch_batch_pair(hipStream_t s, const BatchLaunch &L);
 void launch_batch_contact(hipStream_t s, const BatchLaunch &L, int record);
 
+// ---------------------------------------------------------------------------------------------
+// Per-function entry points (the reference's actions, pair passes, one tool's contact and the
+// leap-frog phases, one kernel each).  The reference keeps some per-particle arrays that the
+// fused step holds only transiently; a particle set allocates them on the first such call.
+// ---------------------------------------------------------------------------------------------
+struct Aux {
+  double *p;     // pressure, N_init
+  double *R;     // artificial stress, 6 per particle (xx yy zz xy yz zx), particle-major
+  double *vder;  // velocity gradient, 9 x stride (m[a][b] at (3a + b) * stride)
+  double *sder;  // stress divergence terms, 9 x stride
+  double *nrm;   // contact normal, 3 x stride
+  double *x0;    // leap-frog start-of-step copies: x y z, vx vy vz, S[6], rho, T (14 x stride)
+  size_t n;      // stride
+};
+enum {
+  ACT_EOS, ACT_ART_STRESS, ACT_JAUMANN, ACT_CONTINUITY, ACT_MOMENTUM, ACT_ADVECTION,
+  ACT_INVALIDATE, ACT_JOHNSON_COOK, ACT_FRIC_HEAT, ACT_BC, ACT_BC_THERMAL, ACT_MOVE_TOOL,
+  ACT_RESET_CONTACT, ACT_LF_INIT
+};
+// vel: the tool velocity of ACT_FRIC_HEAT and ACT_MOVE_TOOL
+void launch_action(hipStream_t s, int which, const State &st, unsigned n_init, const Counters *cnt,
+                   const Consts &c, const Aux &a, const double *vel);
+void launch_lf_advance(hipStream_t s, const State &st, unsigned n_init, const Aux &a, double f);
+void launch_act_records(hipStream_t s, const State &st, unsigned n_init, const Counters *cnt,
+                        const Consts &c, const Aux &a, void *rec, double4 *nl4, double2 *pse2,
+                        double4 *cs4);
+void launch_act_sums(hipStream_t s, const State &st, unsigned n_init, const Counters *cnt,
+                     const Consts &c, const Aux &a, const double *sums, size_t stride,
+                     const int *poison);
+// one tool's contact; with record, per-workgroup force partials (nblocks x MAX_TOOLS x 3, the
+// first three of each used)
+void launch_act_contact(hipStream_t s, const State &st, unsigned n_init, const ToolView &tv,
+                        double dt, const Counters *cnt, const Aux &a, int record, float *partials,
+                        int nblocks);
+
 // triangle binning on the GPU (grid_cpu_tri::get_cells)
 int tool_bin_triangles(hipStream_t s, const Tri *tris, int n_tris, const double bbmin[3],
                        double dx, int nx, int ny, int nz, int **cells_out, int **tri_idx_out,

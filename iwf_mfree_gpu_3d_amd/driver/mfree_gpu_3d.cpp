@@ -3,10 +3,12 @@
 // reference's setup_* functions do (src/Benchmarks/benchmarks_single_grain.cu:674-876), then call
 // leap_frog::step with time advancing by dt after every call, reporting progress and tool forces.
 //
-//   mfree_gpu_3d [preset=1] [scale=1.0] [steps=preset's] [precision=0] [seed=1]
+//   mfree_gpu_3d [preset=1] [scale=1.0] [steps=preset's] [precision=0] [seed=1] [by_actions=0]
 //
 // preset 1: 8,000-particle wedge cut; 2: 1M-particle single grain; 3: 8M three grains.
-// precision: 0 FP64, 1 FP32 + Kahan sums, 2 FP32 + FP64 sums.
+// precision: 0 FP64, 1 FP32 + Kahan sums, 2 FP32 + FP64 sums.  by_actions 1: every step composed
+// call by call from the actions, pair passes and contact calls, as the reference's leap_frog::step
+// makes them (same results).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -24,6 +26,7 @@ int main(int argc, char **argv) {
   const int steps_arg = argc > 3 ? std::atoi(argv[3]) : 0;
   const int precision = argc > 4 ? std::atoi(argv[4]) : MFG_FP64;
   const unsigned long long seed = argc > 5 ? std::strtoull(argv[5], nullptr, 10) : 1ull;
+  const bool by_actions = argc > 6 && std::atoi(argv[6]) != 0;
   try {
     check(mfg_init(0));  // cudaFree(0) in the reference
 
@@ -114,7 +117,10 @@ int main(int argc, char **argv) {
     for (int step = 0; step < last_step; step++) {
       const bool record = step % freq == 0;
       if (step > 0) {
-        stepper.step(&particles, &grid, record && global_tool_forces != nullptr);
+        if (by_actions)
+          stepper.step_by_actions(&particles, &grid, record && global_tool_forces != nullptr);
+        else
+          stepper.step(&particles, &grid, record && global_tool_forces != nullptr);
         active += particles.N;
       }
       if (record && step > 0) {

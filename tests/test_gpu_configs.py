@@ -131,8 +131,9 @@ def test_three_grains_full_size_precision_report(hip_device):
 
 def test_batched_full_configuration(hip_device):
     """The batched form at full size: 256 instances of 5k-200k particles, FP32 + Kahan, 300 steps
-    each.  Every instance ends with finite fields and its particle count; three sampled instances
-    are bit-identical when they run alone."""
+    each, one launch per kernel over all of them.  Every instance ends with finite fields and its
+    particle count; three sampled instances are bit-identical when they run alone (their own
+    single-instance steps)."""
     reqs = serve.sweep_requests(count=256, batched=True, n_min=5e3, n_max=2e5, steps=(300, 300), seed=1,
                                 fp32_share=1.0)
     sample = {0, 100, 255}
@@ -161,7 +162,7 @@ def test_batched_full_configuration(hip_device):
             order = np.argsort(inst.particles.download(_lib.F_ID).astype(np.int64))
             alone[inst.req.rid] = {f: inst.particles.download(f)[order] for f in FIELDS}
 
-        serve.serve([reqs[rid]], on_done=grab)
+        serve.serve([reqs[rid]], on_done=grab, mode="streams")
         for f in FIELDS:
             assert np.isfinite(mix[rid][f]).all()
             np.testing.assert_array_equal(mix[rid][f], alone[rid][f], err_msg=f"request {rid} field {f}")
@@ -208,7 +209,9 @@ def test_served_stream_fp64_request_matches_the_oracle(hip_device):
 def test_cpp_driver_matches_the_python_path(hip_device):
     """The reference-shaped C++ driver (mfree_gpu_3d.cpp's main over the C++ shim) runs the wedge
     cut at half size for 20 steps in its own process; its particle count and its tool-force
-    report of step 20 equal the same run through the Python mirror of the call surface."""
+    report of step 20 equal the same run through the Python mirror of the call surface, and the
+    driver's run with every step composed call by call from the shim's actions, pair passes and
+    contact calls reports the same."""
     exe = os.path.join(ROOT, "iwf_mfree_gpu_3d_amd", "mfree_gpu_3d")
     assert os.path.exists(exe), "the driver is built with the library"
     r = subprocess.run([exe, "1", "0.5", "20", "0", "1"], capture_output=True, text=True, timeout=120)
@@ -225,6 +228,12 @@ def test_cpp_driver_matches_the_python_path(hip_device):
     scenes.run(lf, parts, grid, 1, record_forces=True)
     assert parts.N == n_drv
     np.testing.assert_array_equal(api.global_tool_forces.get_forces_host()[0], f_drv)
+    # the same run with every step composed from the per-function calls through the C++ shim
+    r2 = subprocess.run([exe, "1", "0.5", "20", "0", "1", "1"], capture_output=True, text=True, timeout=120)
+    assert r2.returncode == 0, r2.stderr
+    line2 = [ln for ln in r2.stdout.splitlines() if ln.startswith("000020 of 000020")][0]
+    assert re.search(r"N (\d+)", line2).group(1) == str(n_drv)
+    assert re.search(r"F0 (\S+) (\S+) (\S+)", line2).groups() == re.search(r"F0 (\S+) (\S+) (\S+)", line).groups()
 
 
 @pytest.mark.parametrize("mesh,realscale,feed,steps", [("pin25_reduziert_20062018", False, 4, 300),

@@ -244,7 +244,7 @@ def test_autotuned_batch_same_results(hip_device):
     reqs[1].precision = _lib.MFG_FP64
     table = autotune.tune(reqs, steps=2, warmup=1)
     for (b, p), c in table.choice.items():
-        assert c[:2] in autotune.candidates_for(p) and c[2] in autotune.TILES
+        assert c[:2] in autotune.candidates_for(p) and c[2] in (0,) + autotune.TILES  # 0: automatic kept
     # force choices other than the default so the comparison exercises other kernels / orders
     for (b, p) in table.choice:
         table.choice[(b, p)] = (api.MFG_PAIR_TILED, 0, 8) if p == _lib.MFG_FP64 else (api.MFG_PAIR_LIST, 16, 2)
