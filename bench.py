@@ -73,7 +73,8 @@ def gpu_run(args, preset=None, scale=None, precision=None, steps=None, warmup=No
         f"({scene.params.n_workpiece} workpiece), {sum(t[1].shape[0] for t in scene.tools)} tool triangles, "
         f"built in {time.time() - t0:.1f}s")
     prec = {"fp64": _lib.MFG_FP64, "fp32-kahan": _lib.MFG_FP32_KAHAN, "fp32-wide": _lib.MFG_FP32_WIDE}[precision]
-    parts, grid = scenes.setup_cut(scene, precision=prec, single_sqrt=True, cell_tile=args.cell_tile)
+    parts, grid = scenes.setup_cut(scene, precision=prec, single_sqrt=True, cell_tile=args.cell_tile,
+                                   sort_interval=args.sort_interval)
     lf = api.leap_frog(scene.n, grid.num_cell())
     api.global_time_current = api.global_time_dt
     scenes.run(lf, parts, grid, warmup)
@@ -339,7 +340,8 @@ def sweep(args) -> None:
     if tuning is not None:
         out["autotune"] = {"seconds": tune_s,
                            "choices": [{"bucket_log2_particles": b, "precision": p, "pair_pass": c[0],
-                                        "strip": c[1], "cell_tile": c[2]} for (b, p), c in sorted(tuning.choice.items())],
+                                        "strip": c[1], "cell_tile": c[2], "sort_interval": max(c[3], 1)}
+                                       for (b, p), c in sorted(tuning.choice.items())],
                            "batch_cell_tile": {str(p): t for p, t in sorted(tuning.batch_tile.items())},
                            "batch_ms_per_step": {f"{p}/{t}": s * 1e3 for (p, t), s in sorted(tuning.batch_times.items())}}
     print(json.dumps(out))
@@ -387,6 +389,9 @@ def main():
                          "(batched) or each instance's own kernels on a stream pool (streams)")
     ap.add_argument("--cell-tile", type=int, default=0, choices=[0, 1, 2, 4, 8, 16],
                     help="cells per wavefront of the neighbour pass (0 automatic)")
+    ap.add_argument("--sort-interval", type=int, default=0,
+                    help="opt-in: sort every k-th step of the single-instance runs (0 or 1: every step, "
+                         "the reference's schedule; other values change the results)")
     ap.add_argument("--no-graph", action="store_true",
                     help="batched mode: launch every step's kernels instead of replaying a captured step")
     ap.add_argument("--batched-instances", type=int, default=256,
@@ -484,6 +489,7 @@ def main():
                                              f"orthogonal cut preset {args.preset} (scale {args.scale})"),
                    "particles": res["n_particles"], "active_particles": res["n_active"],
                    "precision": args.precision, "seed": args.seed,
+                   "sort_interval": max(args.sort_interval, 1),
                    "pairs_per_particle": res["pairs"] / max(res["n_active"], 1),
                    "candidates_per_particle": res["cand"] / max(res["n_active"], 1)},
         "roofline": roof,

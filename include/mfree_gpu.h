@@ -116,6 +116,14 @@ typedef struct {
                          0 selects 1, or, where the grid has two or more cells per particle
                          (wide margins around the block), 4 from 256k cells and 16 from 1M.
                          Lists and sums do not depend on it. */
+  int sort_interval;  /* opt-in, no counterpart in the reference (which sorts every step,
+                         src/leap_frog.cu:139-145): mfg_leap_frog_step sorts a particle set on
+                         every k-th step only; in between, the particle order, the cell tables
+                         and the visiting order of the last sort are kept (so a particle is
+                         searched from the cell it was sorted into, the blanking box and the
+                         locked grid's bounds are checked at sorts only, and the neighbour lists
+                         and PSE sums differ from the reference's).  Applies to a locked grid that
+                         does not move; 0 or 1: every step (the reference's results). */
 } mfg_options;
 enum { MFG_PAIR_LIST = 0, MFG_PAIR_TILED = 1 };
 

@@ -14,10 +14,13 @@ form and order first, then the cell tile with them.  For batches (serve mode "ba
 tile is a property of the whole launch: `tune_batched` measures a throwaway batch of the first
 requests of each precision with every tile.
 
-Not a choice here: the re-sort interval.  The reference rebuilds its cell lists from a fresh
-sort every step (src/leap_frog.cu:139-145); sorting less often would change which particles are
-neighbours, and the lists are to stay bit-exact.  Workgroup sizes and waves per SIMD are
-compile-time constants of the kernels (the measured alternatives are documented with them).
+The re-sort interval is offered only when the caller opts in (`sort_intervals`): the reference
+rebuilds its cell lists from a fresh sort every step (src/leap_frog.cu:139-145), and sorting on
+every k-th step only changes the neighbour lists and so the results (the lists are bit-exact with
+the reference's only at interval 1).  Opted in, the last stage measures each interval with the
+chosen form, order and tile, and keeps one only if it is faster.  Workgroup sizes and waves per
+SIMD are compile-time constants of the kernels (the measured alternatives are documented with
+them).
 
 Buckets are powers of two of the particle count; an instance takes the choice of its own
 bucket, else of the nearest tuned one, else the library default.
@@ -36,7 +39,8 @@ CANDIDATES = ((api.MFG_PAIR_LIST, 0), (api.MFG_PAIR_LIST, -1), (api.MFG_PAIR_LIS
               (api.MFG_PAIR_LIST, 16), (api.MFG_PAIR_TILED, 0))
 # cells per wavefront of the neighbour pass; 0 the library's automatic choice
 TILES = (1, 2, 4, 8, 16)
-DEFAULT = (api.MFG_PAIR_LIST, 0, 0)
+# (pair pass, strip, cell tile, sort interval); sort interval 0: every step
+DEFAULT = (api.MFG_PAIR_LIST, 0, 0, 0)
 
 
 def candidates_for(precision: int, candidates=CANDIDATES) -> tuple:
@@ -49,8 +53,9 @@ def bucket(n: int) -> int:
 
 
 class Table:
-    """(bucket, precision) -> (pair pass, strip, cell tile), with the measured seconds per step;
-    precision -> cell tile of a batch."""
+    """(bucket, precision) -> (pair pass, strip, cell tile, sort interval), with the measured
+    seconds per step; precision -> cell tile of a batch.  Choices of three entries (tables
+    written before the sort interval) read as interval 0."""
 
     def __init__(self):
         self.choice = {}
@@ -77,9 +82,9 @@ class Table:
         d = json.loads(text)
         t = Table()
         for row in d["choice"]:
-            t.choice[(row[0], row[1])] = tuple(row[2:])
+            t.choice[(row[0], row[1])] = (tuple(row[2:]) + (0,))[:4]
         for row in d.get("times", []):
-            t.times[(row[0], row[1], tuple(row[2:-1]))] = row[-1]
+            t.times[(row[0], row[1], (tuple(row[2:-1]) + (0,))[:4])] = row[-1]
         for p, tile in d.get("batch_tile", []):
             t.batch_tile[p] = tile
         for p, tile, s in d.get("batch_times", []):
@@ -117,10 +122,11 @@ def measure(req, candidate: tuple, steps: int = 6, warmup: int = 2) -> float:
 
 
 def tune(requests, candidates=CANDIDATES, tiles=TILES, steps: int = 6, warmup: int = 2, measure=measure,
-         log=None) -> Table:
+         log=None, sort_intervals=()) -> Table:
     """One representative request per (bucket, precision) among `requests` (the first seen):
     every form / order candidate measured on it with the automatic cell tile, then every cell
-    tile with the fastest of those; ties go to the earlier candidate."""
+    tile with the fastest of those, then (opt-in, results change) every re-sort interval in
+    `sort_intervals` with the fastest form, order and tile; ties go to the earlier candidate."""
     table = Table()
     reps = {}
     for r in requests:
@@ -129,13 +135,21 @@ def tune(requests, candidates=CANDIDATES, tiles=TILES, steps: int = 6, warmup: i
     for key, r in sorted(reps.items()):
         best = None
         for cand in candidates_for(key[1], candidates):
-            full = (cand[0], cand[1], 0)
+            full = (cand[0], cand[1], 0, 0)
             s = measure(r, full, steps=steps, warmup=warmup)
             table.times[(key[0], key[1], full)] = s
             if best is None or s < best[0]:
                 best = (s, full)
         for tile in tiles:
-            full = (best[1][0], best[1][1], tile)
+            full = (best[1][0], best[1][1], tile, 0)
+            s = measure(r, full, steps=steps, warmup=warmup)
+            table.times[(key[0], key[1], full)] = s
+            if s < best[0]:
+                best = (s, full)
+        for k in sort_intervals:
+            if k <= 1:
+                continue
+            full = (best[1][0], best[1][1], best[1][2], int(k))
             s = measure(r, full, steps=steps, warmup=warmup)
             table.times[(key[0], key[1], full)] = s
             if s < best[0]:
@@ -143,7 +157,7 @@ def tune(requests, candidates=CANDIDATES, tiles=TILES, steps: int = 6, warmup: i
         table.choice[key] = best[1]
         if log:
             log(f"autotune bucket 2^{key[0]} precision {key[1]}: pair pass {best[1][0]}, strip {best[1][1]}, "
-                f"cell tile {best[1][2]} ({best[0] * 1e3:.3f} ms/step)")
+                f"cell tile {best[1][2]}, sort interval {best[1][3] or 1} ({best[0] * 1e3:.3f} ms/step)")
     return table
 
 

@@ -47,7 +47,7 @@ struct Lib {
   hipStream_t streams[MFG_MAX_STREAMS] = {};
   int n_streams = 0;  // streams of the pool (mfg_stream_pool)
   int n_created = 0;  // streams created, >= n_streams
-  mfg_options opt{0, MFG_HEAT_BROOKSHAW, 1, 256, MFG_PAIR_LIST, 0, 0};
+  mfg_options opt{0, MFG_HEAT_BROOKSHAW, 1, 256, MFG_PAIR_LIST, 0, 0, 0};
   // exhaustive contact: closest triangle found by one wavefront per (particle, tool) pair;
   // MFG_LANE_CONTACT=0 in the environment selects the one-thread-per-particle search instead
   bool opt_contact_search = true;
@@ -219,6 +219,10 @@ struct mfg_particles {
   double *cs_res = nullptr;
   unsigned cs_cap = 0;
   int cs_tools = 0;
+  // the last sort (re-sort interval): the grid and its geometry then, and the steps since
+  const mfg_grid *sorted_grid = nullptr;
+  double sorted_geom[7] = {0, 0, 0, 0, 0, 0, 0};
+  int since_sort = 0;
   const State &st() const { return buf[cur]; }
 };
 
@@ -377,6 +381,30 @@ int grid_update_geometry(mfg_grid *g, mfg_particles *p, double kernel_width, dou
 }
 
 // hash (+ blanking), scan, scatter, per-cell fix-up: the sort permutation and cell offsets
+// what decides whether the cell tables of a sort still describe a grid: origin, cell size and
+// cells per axis
+void grid_geometry(const mfg_grid *g, double geom[7]) {
+  for (int k = 0; k < 3; k++) geom[k] = g->bbmin[k];
+  geom[3] = g->dx;
+  geom[4] = (double)g->nx;
+  geom[5] = (double)g->ny;
+  geom[6] = (double)g->nz;
+}
+
+// Whether a step may keep the particle order and cell tables of the last sort (the opt-in
+// re-sort interval): the interval is not over, and the set was last sorted on this grid, which is
+// locked, does not move and has the geometry it had then.
+bool keep_last_sort(const mfg_grid *g, const mfg_particles *p) {
+  const int k = g_lib.opt.sort_interval;
+  if (k <= 1 || p->sorted_grid != g || p->since_sort + 1 >= k || !g->locked) return false;
+  if (g->vel_bbox[0] != 0. || g->vel_bbox[1] != 0. || g->vel_bbox[2] != 0.) return false;
+  double geom[7];
+  grid_geometry(g, geom);
+  for (int a = 0; a < 7; a++)
+    if (geom[a] != p->sorted_geom[a]) return false;
+  return true;
+}
+
 void sort_keys(mfg_grid *g, mfg_particles *p, const mfg_blanking_state *b) {
   GridBuffers gb = grid_buffers(g, p);
   GridInfo gi = g->info();
@@ -407,10 +435,15 @@ void sort_keys(mfg_grid *g, mfg_particles *p, const mfg_blanking_state *b) {
     launch_fixup(g_lib.stream, gi, gb);
   }
   g->scanned = true;
+  p->sorted_grid = g;
+  grid_geometry(g, p->sorted_geom);
+  p->since_sort = 0;
 }
 
-void permute(mfg_grid *g, mfg_particles *p, const Consts &c, int predict) {
+// identity: the particle order stays (a step between sorts): no gather, keys kept
+void permute(mfg_grid *g, mfg_particles *p, const Consts &c, int predict, bool identity = false) {
   GridBuffers gb = grid_buffers(g, p);
+  if (identity) gb.order = nullptr;
   ProfScope ps(predict ? "permute_predict" : "permute");
   int nxt = 1 - p->cur;
   launch_permute_predict_ex(g_lib.stream, p->buf[p->cur], p->buf[nxt], p->N_init, gb, p->cnt, c,
@@ -745,6 +778,8 @@ int mfg_set_options(const mfg_options *o) {
   if (o->cell_tile != 0 && o->cell_tile != 1 && o->cell_tile != 2 && o->cell_tile != 4 &&
       o->cell_tile != 8 && o->cell_tile != 16)
     return fail(MFG_ERR_ARG, "cell tile %d (0, 1, 2, 4, 8 or 16)", o->cell_tile);
+  if (o->sort_interval < 0 || o->sort_interval > 1000)
+    return fail(MFG_ERR_ARG, "sort interval %d (0 to 1000)", o->sort_interval);
   g_lib.opt = *o;
   if (g_lib.opt.max_neighbours <= 0) g_lib.opt.max_neighbours = 256;
   // list rows are read as 16-byte vectors: a multiple of 4 entries, at least 8
@@ -1005,6 +1040,7 @@ int mfg_particle_gpu_snapshot(mfg_particles *p, void *host) {
 int mfg_particle_gpu_restore(mfg_particles *p, const void *host) {
   NEED_LIB();
   if (!p || !host) return fail(MFG_ERR_ARG, "null argument");
+  p->sorted_grid = nullptr;  // another order: the next step sorts
   SnapshotHeader h;
   std::memcpy(&h, host, sizeof(h));
   const size_t n = padded(p->N_init);
@@ -1155,6 +1191,7 @@ int mfg_particle_gpu_download(mfg_particles *p, int f, double *host) {
 int mfg_particle_gpu_upload(mfg_particles *p, int f, const double *host) {
   NEED_LIB();
   if (!p || !host) return fail(MFG_ERR_ARG, "null argument");
+  p->sorted_grid = nullptr;  // new state: the next step sorts
   const unsigned N = p->N_init;
   const State &st = p->st();
   hipStream_t s = g_lib.stream;
@@ -1869,9 +1906,14 @@ int step_impl(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int record,
       if (rc != MFG_OK) return rc;
     }
   }
-  g->scanned = false;
-  sort_keys(g, p, blanking);
-  permute(g, p, c, 1);
+  if (keep_last_sort(g, p)) {
+    p->since_sort++;
+    permute(g, p, c, 1, true);
+  } else {
+    g->scanned = false;
+    sort_keys(g, p, blanking);
+    permute(g, p, c, 1);
+  }
   // pair pass
   const bool pse = c.heat == MFG_HEAT_PSE && c.pse_on;
   {
@@ -2358,6 +2400,7 @@ int mfg_batch_add(mfg_batch *b, mfg_particles *p, mfg_grid *g, mfg_tool *const *
     return fail(MFG_ERR_ARG, "%d tools (max %d)", n_tools, MAX_TOOLS);
   for (int k = 0; k < n_tools; k++)
     if (!tools[k]) return fail(MFG_ERR_ARG, "tool %d is null", k);
+  p->sorted_grid = nullptr;  // batched steps sort on their own: a later single step sorts
   if (forces && forces->n < (unsigned)n_tools)
     return fail(MFG_ERR_ARG, "the force record holds %u tools, the instance has %d", forces->n,
                 n_tools);

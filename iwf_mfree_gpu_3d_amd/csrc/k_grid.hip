@@ -259,8 +259,9 @@ __device__ __forceinline__ void permute_predict(unsigned b, const State &src, co
   const unsigned i = b * BS + threadIdx.x;
   if (i >= n_init) return;
   const unsigned N = cnt->N;
-  const int s = order[i];
-  key_sorted[i] = key[s];
+  // a null order: the particles keep their places (a step between sorts)
+  const int s = order ? order[i] : (int)i;
+  if (order) key_sorted[i] = key[s];
   double x[3] = {src.x[s], src.y[s], src.z[s]};
   double v[3] = {src.vx[s], src.vy[s], src.vz[s]};
   double S[6], St[6];

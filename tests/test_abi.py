@@ -38,6 +38,29 @@ def test_bindings_cover_header():
     assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
 
 
+def test_options_are_validated_without_device():
+    """mfg_set_options rejects a cell tile outside {0, 1, 2, 4, 8, 16} and a sort interval outside
+    [0, 1000] with MFG_ERR_ARG (the previous options stay), and mfg_get_options returns what was
+    set; host only."""
+    lib = _lib.load()
+    before = _lib.options()
+    assert lib.mfg_get_options(C.byref(before)) == 0
+    try:
+        for tile, interval, ok in [(3, 0, False), (4, -1, False), (4, 1001, False), (4, 8, True), (0, 0, True)]:
+            o = _lib.options()
+            C.memmove(C.byref(o), C.byref(before), C.sizeof(o))
+            o.cell_tile, o.sort_interval = tile, interval
+            rc = lib.mfg_set_options(C.byref(o))
+            got = _lib.options()
+            assert lib.mfg_get_options(C.byref(got)) == 0
+            if ok:
+                assert rc == 0 and (got.cell_tile, got.sort_interval) == (tile, interval)
+            else:
+                assert rc == _lib.MFG_ERR_ARG and got.sort_interval in (0, 8)
+    finally:
+        assert lib.mfg_set_options(C.byref(before)) == 0
+
+
 def test_host_only_calls_work_without_device():
     lib = _lib.load()
     assert lib.mfg_version() >= 1

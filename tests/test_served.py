@@ -198,10 +198,10 @@ def test_autotune_picks_the_fastest_per_bucket(tmp_path):
     for b in buckets:
         got = table.choice[(b, _lib.MFG_FP32_KAHAN)]
         if 2 ** (b + 1) <= 5e4:
-            assert got == (api.MFG_PAIR_LIST, -1, 4)
+            assert got == (api.MFG_PAIR_LIST, -1, 4, 0)
         if 2 ** b >= 5e4:
-            assert got == (api.MFG_PAIR_LIST, 16, 8)
-        assert got[:2] in autotune.CANDIDATES and got[2] in autotune.TILES
+            assert got == (api.MFG_PAIR_LIST, 16, 8, 0)
+        assert got[:2] in autotune.CANDIDATES and got[2] in autotune.TILES and got[3] == 0
     assert table.get(10 ** 9, _lib.MFG_FP32_KAHAN) == table.choice[(max(buckets), _lib.MFG_FP32_KAHAN)]
     assert table.get(10, _lib.MFG_FP32_KAHAN) == table.choice[(min(buckets), _lib.MFG_FP32_KAHAN)]
     assert table.get(10 ** 5, _lib.MFG_FP64) == autotune.DEFAULT
@@ -209,6 +209,35 @@ def test_autotune_picks_the_fastest_per_bucket(tmp_path):
     table.save(path)
     back = autotune.Table.load(path)
     assert back.choice == table.choice and back.times == table.times
+
+
+def test_autotune_sort_interval_only_when_opted_in():
+    """The re-sort interval changes results, so the tuner measures it only when asked
+    (`sort_intervals`), last, with the chosen form, order and tile, and keeps an interval only if
+    it is faster; 0 and 1 (every step) are not measured; three-entry choices of older tables read
+    as interval 0."""
+    from iwf_mfree_gpu_3d_amd import autotune
+    reqs = serve.sweep_requests(count=20, batched=True, n_min=5e3, n_max=2e5, steps=(10, 10), seed=5,
+                                fp32_share=1.0)
+    seen = []
+
+    def fake(req, cand, steps, warmup):
+        seen.append(cand)
+        nx, ny, nz = req.lattice()
+        # large instances gain from sorting every 4th step, small ones lose with any interval
+        if cand[3] == 4:
+            return 0.5 if nx * ny * nz >= 5e4 else 3.0
+        return 3.0 if cand[3] > 1 else 1.0
+
+    plain = autotune.tune(reqs, measure=fake)
+    assert all(c[3] == 0 for c in seen) and all(c[3] == 0 for c in plain.choice.values())
+    seen.clear()
+    table = autotune.tune(reqs, measure=fake, sort_intervals=(0, 1, 2, 4))
+    assert sorted({c[3] for c in seen}) == [0, 2, 4]
+    for (b, _), c in table.choice.items():
+        assert c[3] == (4 if 2 ** b >= 5e4 else 0) or 2 ** b < 5e4 <= 2 ** (b + 1)
+    old = autotune.Table.from_json('{"choice": [[12, 1, 0, 16, 4]], "times": []}')
+    assert old.choice[(12, 1)] == (0, 16, 4, 0)
 
 
 def test_autotune_batch_tile_per_precision(tmp_path):
