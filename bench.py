@@ -340,7 +340,8 @@ def sweep(args) -> None:
     if tuning is not None:
         out["autotune"] = {"seconds": tune_s,
                            "choices": [{"bucket_log2_particles": b, "precision": p, "pair_pass": c[0],
-                                        "strip": c[1], "cell_tile": c[2], "sort_interval": max(c[3], 1)}
+                                        "strip": c[1], "cell_tile": c[2], "sort_interval": max(c[3], 1),
+                                        "pair_block": c[4] or 256}
                                        for (b, p), c in sorted(tuning.choice.items())],
                            "batch_cell_tile": {str(p): t for p, t in sorted(tuning.batch_tile.items())},
                            "batch_ms_per_step": {f"{p}/{t}": s * 1e3 for (p, t), s in sorted(tuning.batch_times.items())}}

@@ -124,6 +124,9 @@ typedef struct {
                          locked grid's bounds are checked at sorts only, and the neighbour lists
                          and PSE sums differ from the reference's).  Applies to a locked grid that
                          does not move; 0 or 1: every step (the reference's results). */
+  int pair_block;     /* list pair pass: threads per workgroup (64, 128 or 256; 0 selects 256).
+                         Each wavefront works alone, so the sums do not depend on it; smaller
+                         workgroups spread a small instance over more compute units. */
 } mfg_options;
 enum { MFG_PAIR_LIST = 0, MFG_PAIR_TILED = 1 };
 

@@ -62,7 +62,7 @@ class geom_constants(C.Structure):
 class options(C.Structure):
     _fields_ = [("cspm", C.c_int), ("heat", C.c_int), ("single_sqrt", C.c_int), ("max_neighbours", C.c_int),
                 ("pair_pass", C.c_int), ("pair_strip", C.c_int), ("cell_tile", C.c_int),
-                ("sort_interval", C.c_int)]
+                ("sort_interval", C.c_int), ("pair_block", C.c_int)]
 
 
 class blanking_state(C.Structure):

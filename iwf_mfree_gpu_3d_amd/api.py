@@ -89,17 +89,18 @@ MFG_PAIR_LIST, MFG_PAIR_TILED = 0, 1
 
 def set_options(cspm: bool = False, heat: int = MFG_HEAT_BROOKSHAW, single_sqrt: bool = True,
                 max_neighbours: int = 0, pair_pass: int = MFG_PAIR_LIST, pair_strip: int = 0,
-                cell_tile: int = 0, sort_interval: int = 0) -> None:
+                cell_tile: int = 0, sort_interval: int = 0, pair_block: int = 0) -> None:
     """Switches that are compile-time macros in the reference (src/types.h:32-108), the
     pair-pass kernel (list gather or cell-tiled: the same sums in the same order), the list
     pass's visiting order (strip width in grid columns; 0 default, negative the sorted order),
-    the neighbour pass's cell tile (cells per wavefront: 1, 2, 4, 8; 0 automatic) and the opt-in
+    the neighbour pass's cell tile (cells per wavefront: 1, 2, 4, 8; 0 automatic), the opt-in
     re-sort interval (sort every k-th step on a locked, still grid; 0 or 1 every step, as the
-    reference: other values change the neighbour lists and results)."""
+    reference: other values change the neighbour lists and results) and the list pair pass's
+    workgroup size (64, 128, 256; 0 selects 256; the sums do not depend on it)."""
     _lib.init()
     check(_L().mfg_set_options(C.byref(_lib.options(int(cspm), int(heat), int(single_sqrt), int(max_neighbours),
                                                     int(pair_pass), int(pair_strip), int(cell_tile),
-                                                    int(sort_interval)))))
+                                                    int(sort_interval), int(pair_block)))))
 
 
 def actions_setup_physical_constants(c: phys_constants) -> None:

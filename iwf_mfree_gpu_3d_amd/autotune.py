@@ -7,10 +7,12 @@ Every choice offered leaves results bit-identical, so the tuner only measures ti
   the same order, but in the FP32 modes the two kernels' FMA contraction can differ at rounding
   level;
 - the neighbour pass's cell tile (consecutive grid cells per wavefront: 1, 2, 4, 8, 16), which
-  decides how much an empty cell of a grid with wide margins costs.
+  decides how much an empty cell of a grid with wide margins costs;
+- the list pair pass's workgroup size (64, 128, 256 threads): every wavefront works alone, and
+  smaller workgroups spread a small instance over more compute units.
 
 For instances on a stream pool (serve mode "streams"), `tune` picks per (bucket, precision) the
-form and order first, then the cell tile with them.  For batches (serve mode "batched") the cell
+form and order first, then the cell tile with them, then the pair pass's workgroup size.  For batches (serve mode "batched") the cell
 tile is a property of the whole launch: `tune_batched` measures a throwaway batch of the first
 requests of each precision with every tile.
 
@@ -19,8 +21,8 @@ rebuilds its cell lists from a fresh sort every step (src/leap_frog.cu:139-145),
 every k-th step only changes the neighbour lists and so the results (the lists are bit-exact with
 the reference's only at interval 1).  Opted in, the last stage measures each interval with the
 chosen form, order and tile, and keeps one only if it is faster.  Workgroup sizes and waves per
-SIMD are compile-time constants of the kernels (the measured alternatives are documented with
-them).
+SIMD of the other kernels are compile-time constants (the measured alternatives are documented
+with them).
 
 Buckets are powers of two of the particle count; an instance takes the choice of its own
 bucket, else of the nearest tuned one, else the library default.
@@ -39,8 +41,10 @@ CANDIDATES = ((api.MFG_PAIR_LIST, 0), (api.MFG_PAIR_LIST, -1), (api.MFG_PAIR_LIS
               (api.MFG_PAIR_LIST, 16), (api.MFG_PAIR_TILED, 0))
 # cells per wavefront of the neighbour pass; 0 the library's automatic choice
 TILES = (1, 2, 4, 8, 16)
-# (pair pass, strip, cell tile, sort interval); sort interval 0: every step
-DEFAULT = (api.MFG_PAIR_LIST, 0, 0, 0)
+# threads per workgroup of the list pair pass; 0 the library's default (256)
+BLOCKS = (64, 128, 256)
+# (pair pass, strip, cell tile, sort interval, pair workgroup); sort interval 0: every step
+DEFAULT = (api.MFG_PAIR_LIST, 0, 0, 0, 0)
 
 
 def candidates_for(precision: int, candidates=CANDIDATES) -> tuple:
@@ -53,9 +57,9 @@ def bucket(n: int) -> int:
 
 
 class Table:
-    """(bucket, precision) -> (pair pass, strip, cell tile, sort interval), with the measured
-    seconds per step; precision -> cell tile of a batch.  Choices of three entries (tables
-    written before the sort interval) read as interval 0."""
+    """(bucket, precision) -> (pair pass, strip, cell tile, sort interval, pair workgroup), with
+    the measured seconds per step; precision -> cell tile of a batch.  Shorter choices (tables
+    written before the later entries) read with 0 (the default) for what they lack."""
 
     def __init__(self):
         self.choice = {}
@@ -82,9 +86,9 @@ class Table:
         d = json.loads(text)
         t = Table()
         for row in d["choice"]:
-            t.choice[(row[0], row[1])] = (tuple(row[2:]) + (0,))[:4]
+            t.choice[(row[0], row[1])] = (tuple(row[2:]) + (0, 0))[:5]
         for row in d.get("times", []):
-            t.times[(row[0], row[1], (tuple(row[2:-1]) + (0,))[:4])] = row[-1]
+            t.times[(row[0], row[1], (tuple(row[2:-1]) + (0, 0))[:5])] = row[-1]
         for p, tile in d.get("batch_tile", []):
             t.batch_tile[p] = tile
         for p, tile, s in d.get("batch_times", []):
@@ -122,11 +126,12 @@ def measure(req, candidate: tuple, steps: int = 6, warmup: int = 2) -> float:
 
 
 def tune(requests, candidates=CANDIDATES, tiles=TILES, steps: int = 6, warmup: int = 2, measure=measure,
-         log=None, sort_intervals=()) -> Table:
+         log=None, sort_intervals=(), blocks=BLOCKS) -> Table:
     """One representative request per (bucket, precision) among `requests` (the first seen):
     every form / order candidate measured on it with the automatic cell tile, then every cell
-    tile with the fastest of those, then (opt-in, results change) every re-sort interval in
-    `sort_intervals` with the fastest form, order and tile; ties go to the earlier candidate."""
+    tile with the fastest of those, then every pair-pass workgroup size, then (opt-in, results
+    change) every re-sort interval in `sort_intervals` with the fastest of all the others; ties
+    go to the earlier candidate."""
     table = Table()
     reps = {}
     for r in requests:
@@ -135,13 +140,19 @@ def tune(requests, candidates=CANDIDATES, tiles=TILES, steps: int = 6, warmup: i
     for key, r in sorted(reps.items()):
         best = None
         for cand in candidates_for(key[1], candidates):
-            full = (cand[0], cand[1], 0, 0)
+            full = (cand[0], cand[1], 0, 0, 0)
             s = measure(r, full, steps=steps, warmup=warmup)
             table.times[(key[0], key[1], full)] = s
             if best is None or s < best[0]:
                 best = (s, full)
         for tile in tiles:
-            full = (best[1][0], best[1][1], tile, 0)
+            full = (best[1][0], best[1][1], tile, 0, 0)
+            s = measure(r, full, steps=steps, warmup=warmup)
+            table.times[(key[0], key[1], full)] = s
+            if s < best[0]:
+                best = (s, full)
+        for bs in blocks:
+            full = (best[1][0], best[1][1], best[1][2], 0, int(bs))
             s = measure(r, full, steps=steps, warmup=warmup)
             table.times[(key[0], key[1], full)] = s
             if s < best[0]:
@@ -149,7 +160,7 @@ def tune(requests, candidates=CANDIDATES, tiles=TILES, steps: int = 6, warmup: i
         for k in sort_intervals:
             if k <= 1:
                 continue
-            full = (best[1][0], best[1][1], best[1][2], int(k))
+            full = (best[1][0], best[1][1], best[1][2], int(k), best[1][4])
             s = measure(r, full, steps=steps, warmup=warmup)
             table.times[(key[0], key[1], full)] = s
             if s < best[0]:
@@ -157,7 +168,8 @@ def tune(requests, candidates=CANDIDATES, tiles=TILES, steps: int = 6, warmup: i
         table.choice[key] = best[1]
         if log:
             log(f"autotune bucket 2^{key[0]} precision {key[1]}: pair pass {best[1][0]}, strip {best[1][1]}, "
-                f"cell tile {best[1][2]}, sort interval {best[1][3] or 1} ({best[0] * 1e3:.3f} ms/step)")
+                f"cell tile {best[1][2]}, workgroup {best[1][4] or 256}, sort interval {best[1][3] or 1} "
+                f"({best[0] * 1e3:.3f} ms/step)")
     return table
 
 

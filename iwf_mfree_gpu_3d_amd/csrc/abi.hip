@@ -47,7 +47,7 @@ struct Lib {
   hipStream_t streams[MFG_MAX_STREAMS] = {};
   int n_streams = 0;  // streams of the pool (mfg_stream_pool)
   int n_created = 0;  // streams created, >= n_streams
-  mfg_options opt{0, MFG_HEAT_BROOKSHAW, 1, 256, MFG_PAIR_LIST, 0, 0, 0};
+  mfg_options opt{0, MFG_HEAT_BROOKSHAW, 1, 256, MFG_PAIR_LIST, 0, 0, 0, 0};
   // exhaustive contact: closest triangle found by one wavefront per (particle, tool) pair;
   // MFG_LANE_CONTACT=0 in the environment selects the one-thread-per-particle search instead
   bool opt_contact_search = true;
@@ -780,6 +780,8 @@ int mfg_set_options(const mfg_options *o) {
     return fail(MFG_ERR_ARG, "cell tile %d (0, 1, 2, 4, 8 or 16)", o->cell_tile);
   if (o->sort_interval < 0 || o->sort_interval > 1000)
     return fail(MFG_ERR_ARG, "sort interval %d (0 to 1000)", o->sort_interval);
+  if (o->pair_block != 0 && o->pair_block != 64 && o->pair_block != 128 && o->pair_block != 256)
+    return fail(MFG_ERR_ARG, "pair-pass workgroup size %d (0, 64, 128 or 256)", o->pair_block);
   g_lib.opt = *o;
   if (g_lib.opt.max_neighbours <= 0) g_lib.opt.max_neighbours = 256;
   // list rows are read as 16-byte vectors: a multiple of 4 entries, at least 8
@@ -1932,7 +1934,8 @@ int step_impl(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int record,
                  : p->precision == MFG_FP32_KAHAN ? "pair_fp32_kahan"
                                                   : "pair_fp32_wide");
     launch_pair(s, p->N_init, p->st(), p->rec, p->nbr, p->nl_stride, p->count, c, p->cnt, gi,
-                g->count, p->key_sorted, tiled, p->sums, p->sum_stride, trav, p->cs4);
+                g->count, p->key_sorted, tiled, p->sums, p->sum_stride, trav, p->cs4,
+                g_lib.opt.pair_block);
   }
   {
     ProfScope ps("particle_update");
@@ -2074,7 +2077,8 @@ int mfg_interactions_monaghan(mfg_particles *p, mfg_grid *g) {
   if (rc != MFG_OK) return rc;
   const GridInfo gi = g->info();
   launch_pair(g_lib.stream, p->N_init, p->st(), p->rec, p->nbr, p->nl_stride, p->count, c, p->cnt,
-              gi, g->count, p->key_sorted, 0, p->sums, p->sum_stride, nullptr, p->cs4);
+              gi, g->count, p->key_sorted, 0, p->sums, p->sum_stride, nullptr, p->cs4,
+              g_lib.opt.pair_block);
   launch_act_sums(g_lib.stream, p->st(), p->N_init, p->cnt, c, aux_of(p), p->sums, p->sum_stride,
                   p->poison);
   HIP_TRY(hipGetLastError());

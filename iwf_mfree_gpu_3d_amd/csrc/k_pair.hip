@@ -1177,7 +1177,7 @@ __global__ void __launch_bounds__(BSP, PAIR64_WAVES)
 
 // FP32 list pass with the cooperative gather for the main sum: the 64 lanes of a wavefront run
 // the pair loop together (inactive lanes with an empty list) because every lane helps load.
-template <class Acc>
+template <class Acc, int BS = BSP>
 __device__ __forceinline__ void pair32_coop(unsigned b, unsigned nb, unsigned n_init,
                                             const State &st, const Rec32 *__restrict__ rec,
                                             const int *__restrict__ nbr, int stride,
@@ -1185,7 +1185,7 @@ __device__ __forceinline__ void pair32_coop(unsigned b, unsigned nb, unsigned n_
                                             const Counters *cnt, double *__restrict__ sums,
                                             size_t sstride, const int *__restrict__ trav,
                                             const double4 *__restrict__ cs4) {
-  const unsigned t = xcd_map(b, nb) * BSP + threadIdx.x;
+  const unsigned t = xcd_map(b, nb) * BS + threadIdx.x;
   const unsigned N = cnt->N;
   const bool in_range = t < N && t < n_init;
   const unsigned i = in_range && trav ? (unsigned)trav[t] : t;
@@ -1202,7 +1202,7 @@ __device__ __forceinline__ void pair32_coop(unsigned b, unsigned nb, unsigned n_
     P.cspm_end();
   }
   typedef CoopGather<sizeof(Rec32) / 16, 8, sizeof(Rec32)> CG;
-  __shared__ __align__(16) char s_rec[2][BSP / 64][64 * sizeof(Rec32)];
+  __shared__ __align__(16) char s_rec[2][BS / 64][64 * sizeof(Rec32)];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int nmax = nl;
 #pragma unroll
@@ -1243,14 +1243,16 @@ __device__ __forceinline__ void pair32_coop(unsigned b, unsigned nb, unsigned n_
   if (!act) return;
   P.store(sums, sstride, i);
 }
-template <class Acc>
-__global__ void __launch_bounds__(BSP, PAIR32_WAVES)
+// BS: threads per workgroup (64, 128 or 256; each wavefront works alone, so the sums do not
+// depend on it).  Small instances spread over more compute units with smaller workgroups.
+template <class Acc, int BS>
+__global__ void __launch_bounds__(BS, PAIR32_WAVES)
     k_pair32_coop(unsigned n_init, State st, const Rec32 *__restrict__ rec,
                   const int *__restrict__ nbr, int stride, const int *__restrict__ count,
                   Consts c, const Counters *cnt, double *__restrict__ sums, size_t sstride,
                   const int *__restrict__ trav, const double4 *__restrict__ cs4) {
-  pair32_coop<Acc>(blockIdx.x, gridDim.x, n_init, st, rec, nbr, stride, count, c, cnt, sums,
-                   sstride, trav, cs4);
+  pair32_coop<Acc, BS>(blockIdx.x, gridDim.x, n_init, st, rec, nbr, stride, count, c, cnt, sums,
+                       sstride, trav, cs4);
 }
 
 // FP64 list pass with a cooperative gather through LDS.  Gathered lane by lane, each 16-byte
@@ -1388,6 +1390,7 @@ __device__ __forceinline__ void dma_list_walk(const Rec64 *__restrict__ rec, con
   wave_sync_lds();  // the area is free for the next walk
 }
 
+template <int BS = BSP>
 __device__ __forceinline__ void pair64_dma(unsigned b, unsigned nb, unsigned n_init,
                                            const State &st, const Rec64 *__restrict__ rec,
                                            const int *__restrict__ nbr, int stride,
@@ -1397,7 +1400,7 @@ __device__ __forceinline__ void pair64_dma(unsigned b, unsigned nb, unsigned n_i
                                            const double4 *__restrict__ cs4) {
   typedef Pair64::CRec CRec;
   static_assert(sizeof(Rec64) == 192 && sizeof(CRec) == 64, "record pieces");
-  const unsigned t = xcd_map(b, nb) * BSP + threadIdx.x;
+  const unsigned t = xcd_map(b, nb) * BS + threadIdx.x;
   const unsigned N = cnt->N;
   const bool in_range = t < N && t < n_init;
   const unsigned i = in_range && trav ? (unsigned)trav[t] : t;
@@ -1407,7 +1410,7 @@ __device__ __forceinline__ void pair64_dma(unsigned b, unsigned nb, unsigned n_i
   Pair64 P(c, rec[ic], fl);
   const int nl = act ? count[i] : 0;
   const int *row = nbr + (size_t)ic * stride;
-  __shared__ __align__(16) char s_area[BSP / 64][DMA64_AREA];
+  __shared__ __align__(16) char s_area[BS / 64][DMA64_AREA];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   char *area = s_area[w];
   if (c.cspm) {
@@ -1424,13 +1427,15 @@ __device__ __forceinline__ void pair64_dma(unsigned b, unsigned nb, unsigned n_i
   if (!act) return;
   P.store(sums, sstride, i);
 }
-__global__ void __launch_bounds__(BSP, PAIR64D_WAVES)
+// BS: threads per workgroup, as for k_pair32_coop
+template <int BS>
+__global__ void __launch_bounds__(BS, PAIR64D_WAVES)
     k_pair64_dma(unsigned n_init, State st, const Rec64 *__restrict__ rec,
                  const int *__restrict__ nbr, int stride, const int *__restrict__ count,
                  Consts c, const Counters *cnt, double *__restrict__ sums, size_t sstride,
                  const int *__restrict__ trav, const double4 *__restrict__ cs4) {
-  pair64_dma(blockIdx.x, gridDim.x, n_init, st, rec, nbr, stride, count, c, cnt, sums, sstride,
-             trav, cs4);
+  pair64_dma<BS>(blockIdx.x, gridDim.x, n_init, st, rec, nbr, stride, count, c, cnt, sums,
+                 sstride, trav, cs4);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1701,7 +1706,7 @@ void launch_neighbours(hipStream_t s, const GridInfo &g, const int *offsets, con
 void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *rec, const int *nbr,
                  int nl_stride, const int *count, const Consts &c, Counters *cnt,
                  const GridInfo &g, const int *offsets, const int *key_sorted, int tiled,
-                 double *sums, size_t sstride, const int *trav, const double4 *cs4) {
+                 double *sums, size_t sstride, const int *trav, const double4 *cs4, int block) {
   if (tiled) {
     const dim3 grid((unsigned)((long)g.nx * g.ny)), block(64);
     if (c.precision == MFG_FP64)
@@ -1718,21 +1723,31 @@ void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *re
                          cnt, sums, sstride);
     return;
   }
-  const dim3 grid(nb(n_init, BSP)), block(BSP);
-  if (c.precision == MFG_FP64 && PAIR64_COOP)
-    hipLaunchKernelGGL(k_pair64_dma, grid, block, 0, s, n_init, st, (const Rec64 *)rec, nbr,
-                       nl_stride, count, c, cnt, sums, sstride, trav, cs4);
-  else if (c.precision == MFG_FP64)
-    hipLaunchKernelGGL((k_pair_list<Pair64>), grid, block, 0, s, n_init, st, (const Rec64 *)rec,
-                       nbr, nl_stride, count, c, cnt, sums, sstride, trav);
-  else if (c.precision == MFG_FP32_KAHAN)
-    hipLaunchKernelGGL((k_pair32_coop<AccKahan>), grid, block, 0, s, n_init, st,
-                       (const Rec32 *)rec, nbr, nl_stride, count, c, cnt, sums, sstride, trav,
-                       cs4);
+  const int bs = block == 64 || block == 128 ? block : BSP;
+  const dim3 grid(nb(n_init, bs)), blk(bs);
+  auto launch = [&](auto bs_c) {
+    constexpr int BS = decltype(bs_c)::value;
+    if (c.precision == MFG_FP64 && PAIR64_COOP)
+      hipLaunchKernelGGL(k_pair64_dma<BS>, grid, blk, 0, s, n_init, st, (const Rec64 *)rec, nbr,
+                         nl_stride, count, c, cnt, sums, sstride, trav, cs4);
+    else if (c.precision == MFG_FP64)
+      hipLaunchKernelGGL((k_pair_list<Pair64>), dim3(nb(n_init, BSP)), dim3(BSP), 0, s, n_init, st,
+                         (const Rec64 *)rec, nbr, nl_stride, count, c, cnt, sums, sstride, trav);
+    else if (c.precision == MFG_FP32_KAHAN)
+      hipLaunchKernelGGL((k_pair32_coop<AccKahan, BS>), grid, blk, 0, s, n_init, st,
+                         (const Rec32 *)rec, nbr, nl_stride, count, c, cnt, sums, sstride, trav,
+                         cs4);
+    else
+      hipLaunchKernelGGL((k_pair32_coop<AccWide, BS>), grid, blk, 0, s, n_init, st,
+                         (const Rec32 *)rec, nbr, nl_stride, count, c, cnt, sums, sstride, trav,
+                         cs4);
+  };
+  if (bs == 64)
+    launch(std::integral_constant<int, 64>{});
+  else if (bs == 128)
+    launch(std::integral_constant<int, 128>{});
   else
-    hipLaunchKernelGGL((k_pair32_coop<AccWide>), grid, block, 0, s, n_init, st,
-                       (const Rec32 *)rec, nbr, nl_stride, count, c, cnt, sums, sstride, trav,
-                       cs4);
+    launch(std::integral_constant<int, BSP>{});
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -199,7 +199,8 @@ void launch_neighbours(hipStream_t s, const GridInfo &g, const int *offsets, con
 void launch_pair(hipStream_t s, unsigned n_init, const State &st, const void *rec, const int *nbr,
                  int nl_stride, const int *count, const Consts &c, Counters *cnt,
                  const GridInfo &g, const int *offsets, const int *key_sorted, int tiled,
-                 double *sums, size_t sum_stride, const int *trav, const double4 *cs4);
+                 double *sums, size_t sum_stride, const int *trav, const double4 *cs4,
+                 int block);
 // visiting order of the list pair pass: strips of `strip` grid columns along y, x-row by x-row
 // inside a strip (trav[t] = particle visited at position t < N); seg_pos holds
 // pair_order_segments(g, strip) ints

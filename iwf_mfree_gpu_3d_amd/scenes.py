@@ -182,13 +182,14 @@ def engagement_advance(scene: Scene, samples: int = 64) -> float:
 
 def setup_cut(scene: Scene, precision: int = MFG_FP64, single_sqrt: bool = True, max_neighbours: int = 0,
               record_forces: bool = True, pair_pass: int = 0, pair_strip: int = 0, cell_tile: int = 0,
-              sort_interval: int = 0):
+              sort_interval: int = 0, pair_block: int = 0):
     """Constants, tools, grid, blanking and particles of `scene` on the device; sets the process
     globals like the reference set-ups do.  Returns (particles, grid)."""
     p = scene.params
     api.reset_globals()
     api.set_options(cspm=bool(p.cspm), heat=int(p.heat), single_sqrt=single_sqrt, max_neighbours=max_neighbours,
-                    pair_pass=pair_pass, pair_strip=pair_strip, cell_tile=cell_tile, sort_interval=sort_interval)
+                    pair_pass=pair_pass, pair_strip=pair_strip, cell_tile=cell_tile, sort_interval=sort_interval,
+                    pair_block=pair_block)
     api.actions_setup_corrector_constants(p.corr)
     api.actions_setup_physical_constants(p.phys)
     api.actions_setup_johnson_cook_constants(p.joco)

@@ -103,12 +103,12 @@ class Instance:
 
     DX = 5e-4  # lattice spacing [cm]
 
-    def __init__(self, req: Request, stream: int = 0, choice: tuple = (api.MFG_PAIR_LIST, 0, 0, 0),
+    def __init__(self, req: Request, stream: int = 0, choice: tuple = (api.MFG_PAIR_LIST, 0, 0, 0, 0),
                  max_neighbours: int = 0):
         self.req = req
         self.stream = stream
-        # (pair pass, strip, cell tile, sort interval) the instance's steps run with
-        self.choice = (tuple(choice) + (0, 0, 0, 0))[:4]
+        # (pair pass, strip, cell tile, sort interval, pair workgroup) the instance's steps run with
+        self.choice = (tuple(choice) + (0, 0, 0, 0, 0))[:5]
         api.select_stream(stream)
         nx, ny, nz = req.lattice()
         cfg = _lib.cut_config()
@@ -133,7 +133,8 @@ class Instance:
         api.constants_reset()
         api.set_options(cspm=bool(p.cspm), heat=int(p.heat), single_sqrt=True,
                         pair_pass=int(self.choice[0]), pair_strip=int(self.choice[1]),
-                        cell_tile=int(self.choice[2]), sort_interval=int(self.choice[3]))
+                        cell_tile=int(self.choice[2]), sort_interval=int(self.choice[3]),
+                        pair_block=int(self.choice[4]))
         api.actions_setup_corrector_constants(p.corr)
         api.actions_setup_physical_constants(p.phys)
         api.actions_setup_johnson_cook_constants(p.joco)

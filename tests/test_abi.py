@@ -39,22 +39,23 @@ def test_bindings_cover_header():
 
 
 def test_options_are_validated_without_device():
-    """mfg_set_options rejects a cell tile outside {0, 1, 2, 4, 8, 16} and a sort interval outside
-    [0, 1000] with MFG_ERR_ARG (the previous options stay), and mfg_get_options returns what was
-    set; host only."""
+    """mfg_set_options rejects a cell tile outside {0, 1, 2, 4, 8, 16}, a sort interval outside
+    [0, 1000] and a pair-pass workgroup size outside {0, 64, 128, 256} with MFG_ERR_ARG (the
+    previous options stay), and mfg_get_options returns what was set; host only."""
     lib = _lib.load()
     before = _lib.options()
     assert lib.mfg_get_options(C.byref(before)) == 0
     try:
-        for tile, interval, ok in [(3, 0, False), (4, -1, False), (4, 1001, False), (4, 8, True), (0, 0, True)]:
+        for tile, interval, block, ok in [(3, 0, 0, False), (4, -1, 0, False), (4, 1001, 0, False),
+                                          (4, 0, 96, False), (4, 8, 128, True), (0, 0, 0, True)]:
             o = _lib.options()
             C.memmove(C.byref(o), C.byref(before), C.sizeof(o))
-            o.cell_tile, o.sort_interval = tile, interval
+            o.cell_tile, o.sort_interval, o.pair_block = tile, interval, block
             rc = lib.mfg_set_options(C.byref(o))
             got = _lib.options()
             assert lib.mfg_get_options(C.byref(got)) == 0
             if ok:
-                assert rc == 0 and (got.cell_tile, got.sort_interval) == (tile, interval)
+                assert rc == 0 and (got.cell_tile, got.sort_interval, got.pair_block) == (tile, interval, block)
             else:
                 assert rc == _lib.MFG_ERR_ARG and got.sort_interval in (0, 8)
     finally:

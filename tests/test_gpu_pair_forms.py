@@ -23,10 +23,10 @@ FIELDS = [_lib.F_POS, _lib.F_VEL, _lib.F_RHO, _lib.F_S, _lib.F_T, _lib.F_EPS_PL,
           _lib.F_VEL_T, _lib.F_RHO_T, _lib.F_S_T, _lib.F_T_T, _lib.F_FC]
 
 
-def run_case(preset, scale, seed, precision, pair_pass, steps, pair_strip=0, cell_tile=0):
+def run_case(preset, scale, seed, precision, pair_pass, steps, pair_strip=0, cell_tile=0, pair_block=0):
     sc = scenes.Scene(preset, scale, seed)
     parts, grid = scenes.setup_cut(sc, precision=precision, pair_pass=pair_pass, pair_strip=pair_strip,
-                                   cell_tile=cell_tile)
+                                   cell_tile=cell_tile, pair_block=pair_block)
     api.global_time_current = sc.params.dt
     lf = api.leap_frog(sc.n, grid.num_cell())
     scenes.run(lf, parts, grid, steps)
@@ -88,3 +88,17 @@ def test_cell_tile_does_not_change_results(hip_device, case, tile):
     assert a["N"] == b["N"]
     for f in FIELDS:
         np.testing.assert_array_equal(b[f], a[f], err_msg=f"field {f}")
+
+
+@pytest.mark.parametrize("block", [64, 128])
+@pytest.mark.parametrize("precision", [_lib.MFG_FP64, _lib.MFG_FP32_KAHAN, _lib.MFG_FP32_WIDE],
+                         ids=["fp64", "kahan", "wide"])
+def test_pair_block_does_not_change_results(hip_device, precision, block):
+    """The list pair pass with 64- or 128-thread workgroups: every field bit-identical to the
+    default 256 after 30 steps of the single-grain thermal-contact cut (each wavefront works
+    alone)."""
+    a = run_case(2, 0.2, 3, precision, api.MFG_PAIR_LIST, 30)
+    b = run_case(2, 0.2, 3, precision, api.MFG_PAIR_LIST, 30, pair_block=block)
+    assert a["N"] == b["N"]
+    for f in FIELDS:
+        np.testing.assert_array_equal(a[f], b[f], err_msg=f"field {f}")

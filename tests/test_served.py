@@ -170,9 +170,10 @@ def test_served_request_reports_device_errors(hip_device):
 
 def test_autotune_picks_the_fastest_per_bucket(tmp_path):
     """Per (size bucket, precision) the first request seen is measured with every form / order
-    candidate, then with every cell tile under the fastest of those; the fastest wins (ties: the
-    earlier candidate); instances of untuned sizes take the nearest bucket's choice, untuned
-    precisions the default; the table survives a JSON round trip."""
+    candidate, then with every cell tile under the fastest of those, then with every pair-pass
+    workgroup size; the fastest wins (ties: the earlier candidate); instances of untuned sizes
+    take the nearest bucket's choice, untuned precisions the default; the table survives a JSON
+    round trip."""
     from iwf_mfree_gpu_3d_amd import autotune
     reqs = serve.sweep_requests(count=40, batched=True, n_min=5e3, n_max=2e5, steps=(10, 10), seed=4,
                                 fp32_share=1.0)
@@ -187,20 +188,22 @@ def test_autotune_picks_the_fastest_per_bucket(tmp_path):
         t = 2.0 if cand[:2] != fast else 1.0
         if cand[2] == (4 if small else 8):
             t -= 0.5
+        if small and cand[4] == 64:
+            t -= 0.25
         return t
 
     table = autotune.tune(reqs, measure=fake)
     buckets = sorted({autotune.bucket(np.prod(r.lattice())) for r in reqs})
     # FP32: the visiting orders only (the tiled form is not bit-identical there), then the tiles
-    assert len(seen) == len(buckets) * (4 + len(autotune.TILES))
+    assert len(seen) == len(buckets) * (4 + len(autotune.TILES) + len(autotune.BLOCKS))
     assert all(c[0] == api.MFG_PAIR_LIST for _, c in seen)
     assert len(autotune.candidates_for(_lib.MFG_FP64)) == len(autotune.CANDIDATES)
     for b in buckets:
         got = table.choice[(b, _lib.MFG_FP32_KAHAN)]
         if 2 ** (b + 1) <= 5e4:
-            assert got == (api.MFG_PAIR_LIST, -1, 4, 0)
+            assert got == (api.MFG_PAIR_LIST, -1, 4, 0, 64)
         if 2 ** b >= 5e4:
-            assert got == (api.MFG_PAIR_LIST, 16, 8, 0)
+            assert got == (api.MFG_PAIR_LIST, 16, 8, 0, 0)
         assert got[:2] in autotune.CANDIDATES and got[2] in autotune.TILES and got[3] == 0
     assert table.get(10 ** 9, _lib.MFG_FP32_KAHAN) == table.choice[(max(buckets), _lib.MFG_FP32_KAHAN)]
     assert table.get(10, _lib.MFG_FP32_KAHAN) == table.choice[(min(buckets), _lib.MFG_FP32_KAHAN)]
@@ -237,7 +240,7 @@ def test_autotune_sort_interval_only_when_opted_in():
     for (b, _), c in table.choice.items():
         assert c[3] == (4 if 2 ** b >= 5e4 else 0) or 2 ** b < 5e4 <= 2 ** (b + 1)
     old = autotune.Table.from_json('{"choice": [[12, 1, 0, 16, 4]], "times": []}')
-    assert old.choice[(12, 1)] == (0, 16, 4, 0)
+    assert old.choice[(12, 1)] == (0, 16, 4, 0, 0)
 
 
 def test_autotune_batch_tile_per_precision(tmp_path):
