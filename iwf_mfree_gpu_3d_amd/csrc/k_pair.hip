@@ -1331,11 +1331,13 @@ typedef __attribute__((address_space(3))) void lds_void;
 
 // Walks the lists of a wavefront: step k gives each lane with k < nl the first P pieces of its
 // neighbour k.  Step k + 1's records and the index window of steps k + 2 .. k + 5 are loaded
-// while step k's term runs.
+// while step k's term runs.  The records are read through a buffer descriptor whose stride is
+// the record size: the load unit forms record index x stride + piece offset (no 64-bit address
+// arithmetic per load), and an index beyond n_rec reads zeros instead of faulting.
 template <class Plan, class Term>
-__device__ __forceinline__ void dma_list_walk(const Rec64 *__restrict__ rec, const int *row,
-                                              int nl, int nmax, unsigned ic, int lane,
-                                              char *area, Term &&term) {
+__device__ __forceinline__ void dma_list_walk(const Rec64 *__restrict__ rec, unsigned n_rec,
+                                              const int *row, int nl, int nmax, unsigned ic,
+                                              int lane, char *area, Term &&term) {
   constexpr int P = Plan::P, NI = Plan::NI;
   static_assert(NI * DMA_BLOCK <= DMA64_WINDOWS, "area");
   typedef int iv4 __attribute__((ext_vector_type(4)));
@@ -1350,17 +1352,18 @@ __device__ __forceinline__ void dma_list_walk(const Rec64 *__restrict__ rec, con
     const int q = k & 3;
     return q == 0 ? wc.x : q == 1 ? wc.y : q == 2 ? wc.z : wc.w;
   };
+  // word 3: the untyped 32-bit data format the load-to-LDS path needs on gfx9-family parts
+  const __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void *)rec, (short)sizeof(Rec64), (int)n_rec, 0x00020000);
   auto issue = [&](int k) {
     const int j = idx_at(k);
     int jm[NI];
 #pragma unroll
     for (int q = 0; q < NI; q++) jm[q] = __shfl(j, plan.src_lane(q, lane), 64);
 #pragma unroll
-    for (int q = 0; q < NI; q++) {
-      const char *src = (const char *)rec + (size_t)jm[q] * sizeof(Rec64) + plan.off(q);
-      __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)(area + DMA_BLOCK * q), 16,
-                                       0, 0);
-    }
+    for (int q = 0; q < NI; q++)
+      __builtin_amdgcn_struct_ptr_buffer_load_lds(rsrc, (lds_void *)(area + DMA_BLOCK * q), 16,
+                                                  jm[q], plan.off(q), 0, 0, 0);
   };
   if (nmax > 0) issue(0);
   for (int k = 0; k < nmax; k++) {
@@ -1422,7 +1425,7 @@ __device__ __forceinline__ void pair64_dma(unsigned b, unsigned nb, unsigned n_i
   int nmax = nl;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o, 64));
-  dma_list_walk<MainPlan>(rec, row, nl, nmax, ic, lane, area,
+  dma_list_walk<MainPlan>(rec, n_init, row, nl, nmax, ic, lane, area,
                     [&](const auto &pc) { P.term(c, __builtin_bit_cast(Rec64, pc)); });
   if (!act) return;
   P.store(sums, sstride, i);
