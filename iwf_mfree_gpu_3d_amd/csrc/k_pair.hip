@@ -1279,8 +1279,9 @@ __global__ void __launch_bounds__(BS, PAIR32_WAVES)
 constexpr int DMA_BLOCK = 1280;  // 64 x 16 bytes, 256-byte aligned blocks
 struct MainPlan {
   static constexpr int P = 12, NI = 13;
-  // per instruction: source lane (bits 0-7) and piece offset (bits 8-15), two per word, kept in
-  // LDS (word m of lane l at codes + 4 (64 m + l)) to spare the registers
+  // per instruction: the source lane's byte address for ds_bpermute (4 x lane, bits 0-7) and the
+  // piece offset (bits 8-15), two per word, kept in LDS (word m of lane l at codes + 4 (64 m + l))
+  // to spare the registers
   unsigned *codes_;
   int lane_;
   __device__ static int quad(int q, int rr, int lam) {
@@ -1304,7 +1305,7 @@ struct MainPlan {
         r = 5 * q;
         pc = 0;
       }
-      const unsigned code = (unsigned)r | ((unsigned)(16 * pc) << 8);
+      const unsigned code = (unsigned)(4 * r) | ((unsigned)(16 * pc) << 8);
       if (q & 1) {
         codes_[64 * (q >> 1) + lane] = word | code << 16;
       } else {
@@ -1314,7 +1315,7 @@ struct MainPlan {
     }
   }
   __device__ unsigned code(int q) const { return codes_[64 * (q >> 1) + lane_] >> (16 * (q & 1)); }
-  __device__ int src_lane(int q, int) const { return code(q) & 0xff; }
+  __device__ int src_addr(int q) const { return code(q) & 0xff; }
   __device__ int off(int q) const { return (code(q) >> 8) & 0xff; }
   __device__ static int read_pos(int L, int p) {
     const int q = L / 5, rr = L - 5 * q;
@@ -1359,7 +1360,7 @@ __device__ __forceinline__ void dma_list_walk(const Rec64 *__restrict__ rec, uns
     const int j = idx_at(k);
     int jm[NI];
 #pragma unroll
-    for (int q = 0; q < NI; q++) jm[q] = __shfl(j, plan.src_lane(q, lane), 64);
+    for (int q = 0; q < NI; q++) jm[q] = __builtin_amdgcn_ds_bpermute(plan.src_addr(q), j);
 #pragma unroll
     for (int q = 0; q < NI; q++)
       __builtin_amdgcn_struct_ptr_buffer_load_lds(rsrc, (lds_void *)(area + DMA_BLOCK * q), 16,
@@ -1414,7 +1415,9 @@ __device__ __forceinline__ void pair64_dma(unsigned b, unsigned nb, unsigned n_i
   const int nl = act ? count[i] : 0;
   const int *row = nbr + (size_t)ic * stride;
   __shared__ __align__(16) char s_area[BS / 64][DMA64_AREA];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // the wave index as a scalar: the LDS destinations of the record loads (M0) are then formed
+  // by the scalar unit
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   char *area = s_area[w];
   if (c.cspm) {
     if (P.cspm_sum())
