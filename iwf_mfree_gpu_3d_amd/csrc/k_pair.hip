@@ -560,15 +560,14 @@ __device__ __forceinline__ void gather_loop(const int *row, int n, Load &&load, 
   }
 }
 
-// Cooperative gather.  The 64 lanes of a wavefront each need one neighbour record per step.
-// Loaded lane by lane, every 16-byte piece of every record is a separate L1 access, and the L1
-// serves about one per cycle.  Here groups of G lanes load the P pieces of one record each (one
-// piece per lane and instruction), so every load instruction reads 64 / G whole records from
-// contiguous lines; the pieces then go through LDS to the lane that needs the record.  The LDS
-// slot of a record is S bytes (S / 16 odd multiples of four dwords keep the 16-lane phases of a
-// 16-byte LDS access on distinct banks).
+// A 16-byte piece of a record (the unit of the cooperative gathers through LDS below).
 typedef unsigned piece16 __attribute__((ext_vector_type(4)));
 
+// Cooperative gather through registers: groups of G lanes load the P pieces of one record each
+// (one piece per lane and instruction), so every load instruction reads 64 / G whole records
+// from contiguous lines; the pieces then go through LDS to the lane that needs the record.  The
+// LDS slot of a record is S bytes (S / 16 odd multiples of four dwords keep the 16-lane phases of
+// a 16-byte LDS access on distinct banks).
 template <int P, int G, int S>
 struct CoopGather {
   static_assert(P <= G && (G & (G - 1)) == 0 && G <= 64, "pieces per record");
@@ -1175,85 +1174,6 @@ __global__ void __launch_bounds__(BSP, PAIR64_WAVES)
   P.store(sums, sstride, i);
 }
 
-// FP32 list pass with the cooperative gather for the main sum: the 64 lanes of a wavefront run
-// the pair loop together (inactive lanes with an empty list) because every lane helps load.
-template <class Acc, int BS = BSP>
-__device__ __forceinline__ void pair32_coop(unsigned b, unsigned nb, unsigned n_init,
-                                            const State &st, const Rec32 *__restrict__ rec,
-                                            const int *__restrict__ nbr, int stride,
-                                            const int *__restrict__ count, const Consts &c,
-                                            const Counters *cnt, double *__restrict__ sums,
-                                            size_t sstride, const int *__restrict__ trav,
-                                            const double4 *__restrict__ cs4) {
-  const unsigned t = xcd_map(b, nb) * BS + threadIdx.x;
-  const unsigned N = cnt->N;
-  const bool in_range = t < N && t < n_init;
-  const unsigned i = in_range && trav ? (unsigned)trav[t] : t;
-  const uint32_t fl = in_range ? st.flags[i] : FL_BLANKED;
-  const bool act = in_range && !(fl & FL_BLANKED);
-  const unsigned ic = act ? i : 0u;
-  Pair32<Acc> P(c, rec[ic], fl);
-  const int nl = act ? count[i] : 0;
-  const int *row = nbr + (size_t)ic * stride;
-  if (c.cspm) {
-    if (P.cspm_sum())
-      gather_loop(row, nl, [&](int j) { return cs4[j]; },
-                  [&](const double4 &g) { P.cspm_term(c, g); });
-    P.cspm_end();
-  }
-  typedef CoopGather<sizeof(Rec32) / 16, 8, sizeof(Rec32)> CG;
-  __shared__ __align__(16) char s_rec[2][BS / 64][64 * sizeof(Rec32)];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int nmax = nl;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o, 64));
-  typedef int iv4 __attribute__((ext_vector_type(4)));
-  iv4 win = {0, 0, 0, 0};
-  // neighbour of this lane at step k (its own particle past the end of its list); k ascends
-  auto idx_at = [&](int k) -> int {
-    if ((k & 3) == 0 && k < nl) win = *(const iv4 *)(row + k);
-    if (k >= nl) return (int)ic;
-    const int q = k & 3;
-    return q == 0 ? win.x : q == 1 ? win.y : q == 2 ? win.z : win.w;
-  };
-  piece16 v[8];
-  if (nmax > 0) {
-    CG::load((const char *)rec, sizeof(Rec32), idx_at(0), lane, v);
-    CG::store(s_rec[0][w], lane, v);
-    wave_sync_lds();
-  }
-  for (int k = 0; k < nmax; k++) {
-    const bool more = k + 1 < nmax;
-    if (more) CG::load((const char *)rec, sizeof(Rec32), idx_at(k + 1), lane, v);
-    if (k < nl) {
-      // the record as seven 16-byte reads: at the 112-byte slot stride the 8 lanes a 16-byte
-      // read serves per cycle fall on distinct banks, while 4- and 8-byte field reads would put
-      // 4 lanes on each bank
-      struct Pieces {
-        piece16 p[sizeof(Rec32) / 16];
-      } pc;
-      const char *src = s_rec[k & 1][w] + lane * sizeof(Rec32);
-#pragma unroll
-      for (int q = 0; q < (int)(sizeof(Rec32) / 16); q++) pc.p[q] = *(const piece16 *)(src + 16 * q);
-      P.term(c, __builtin_bit_cast(Rec32, pc));
-    }
-    if (more) CG::store(s_rec[(k + 1) & 1][w], lane, v);
-    wave_sync_lds();
-  }
-  if (!act) return;
-  P.store(sums, sstride, i);
-}
-// BS: threads per workgroup (64, 128 or 256; each wavefront works alone, so the sums do not
-// depend on it).  Small instances spread over more compute units with smaller workgroups.
-template <class Acc, int BS>
-__global__ void __launch_bounds__(BS, PAIR32_WAVES)
-    k_pair32_coop(unsigned n_init, State st, const Rec32 *__restrict__ rec,
-                  const int *__restrict__ nbr, int stride, const int *__restrict__ count,
-                  Consts c, const Counters *cnt, double *__restrict__ sums, size_t sstride,
-                  const int *__restrict__ trav, const double4 *__restrict__ cs4) {
-  pair32_coop<Acc, BS>(blockIdx.x, gridDim.x, n_init, st, rec, nbr, stride, count, c, cnt, sums,
-                       sstride, trav, cs4);
-}
 
 // FP64 list pass with a cooperative gather through LDS.  Gathered lane by lane, each 16-byte
 // piece of each neighbour record is its own L1 access, and the L1 serves about one access per
@@ -1278,7 +1198,11 @@ __global__ void __launch_bounds__(BS, PAIR32_WAVES)
 // instruction, on bank group 4 (quad & 3) + ((p - rot) & 3) = 4 ((L + lam) & 3) + ((p - rot) & 3).
 constexpr int DMA_BLOCK = 1280;  // 64 x 16 bytes, 256-byte aligned blocks
 struct MainPlan {
-  static constexpr int P = 12, NI = 13;
+  typedef Rec64 Rec;
+  static constexpr int P = 12, NI = 13, BLOCK = DMA_BLOCK;
+  // area: NI load blocks, the index window (64 x 16 bytes), the load codes
+  static constexpr int WINDOWS = NI * BLOCK, CODES = WINDOWS + 64 * 16;
+  static constexpr int AREA = CODES + 64 * 4 * ((NI + 1) / 2);
   // per instruction: the source lane's byte address for ds_bpermute (4 x lane, bits 0-7) and the
   // piece offset (bits 8-15), two per word, kept in LDS (word m of lane l at codes + 4 (64 m + l))
   // to spare the registers
@@ -1289,7 +1213,8 @@ struct MainPlan {
     const int rb = (c - 5 * q + 1) & 3;  // the one rr (mod 4) with no line of class c
     return c + 4 * (rr - (rr > rb ? 1 : 0));
   }
-  __device__ void prepare(int lane) {
+  __device__ void prepare(char *area, int lane) {
+    codes_ = (unsigned *)(area + CODES);
     lane_ = lane;
     const int g = lane >> 2, e = lane & 3, c = g & 3, k = g >> 2;
     unsigned word = 0;
@@ -1322,9 +1247,30 @@ struct MainPlan {
     return DMA_BLOCK * q + 64 * quad(q, rr, p >> 2) + 16 * ((p - ((L >> 2) & 3)) & 3);
   }
 };
-constexpr int DMA64_WINDOWS = MainPlan::NI * DMA_BLOCK;  // the index windows follow the blocks
-constexpr int DMA64_CODES = DMA64_WINDOWS + 64 * 16;     // then the main walk's load codes
-constexpr int DMA64_AREA = DMA64_CODES + 64 * 4 * ((MainPlan::NI + 1) / 2);
+constexpr int DMA64_AREA = MainPlan::AREA;
+// FP32 records (112 bytes, seven pieces): the step's 64 records lie in LDS one after the other
+// (record L at 112 L), as seven 1 KB blocks: load instruction q fetches piece g = 64 q + lane of
+// that image (record g / 7, piece g mod 7), so every lane loads and no piece is fetched twice.
+// At the 112-byte record stride the 16-byte reads of a lane's own record fall on distinct banks
+// for the lanes one read phase serves.
+struct Plan32 {
+  typedef Rec32 Rec;
+  static constexpr int P = 7, NI = 7, BLOCK = 64 * 16;
+  static constexpr int WINDOWS = NI * BLOCK, AREA = WINDOWS + 64 * 16;
+  static_assert(sizeof(Rec32) == 16 * P && NI * BLOCK == 64 * (int)sizeof(Rec32), "FP32 record image");
+  int src_[NI], off_[NI];
+  __device__ void prepare(char *, int lane) {
+#pragma unroll
+    for (int q = 0; q < NI; q++) {
+      const int g = 64 * q + lane, r = g / P;
+      src_[q] = 4 * r;
+      off_[q] = 16 * (g - P * r);
+    }
+  }
+  __device__ int src_addr(int q) const { return src_[q]; }
+  __device__ int off(int q) const { return off_[q]; }
+  __device__ static int read_pos(int L, int p) { return (int)sizeof(Rec32) * L + 16 * p; }
+};
 #ifndef PAIR64D_WAVES
 #define PAIR64D_WAVES 2
 #endif
@@ -1336,16 +1282,16 @@ typedef __attribute__((address_space(3))) void lds_void;
 // the record size: the load unit forms record index x stride + piece offset (no 64-bit address
 // arithmetic per load), and an index beyond n_rec reads zeros instead of faulting.
 template <class Plan, class Term>
-__device__ __forceinline__ void dma_list_walk(const Rec64 *__restrict__ rec, unsigned n_rec,
-                                              const int *row, int nl, int nmax, unsigned ic,
-                                              int lane, char *area, Term &&term) {
+__device__ __forceinline__ void dma_list_walk(const typename Plan::Rec *__restrict__ rec,
+                                              unsigned n_rec, const int *row, int nl, int nmax,
+                                              unsigned ic, int lane, char *area, Term &&term) {
+  typedef typename Plan::Rec Rec;
   constexpr int P = Plan::P, NI = Plan::NI;
-  static_assert(NI * DMA_BLOCK <= DMA64_WINDOWS, "area");
+  static_assert(NI * Plan::BLOCK <= Plan::WINDOWS, "area");
   typedef int iv4 __attribute__((ext_vector_type(4)));
-  char *wslot = area + DMA64_WINDOWS;
+  char *wslot = area + Plan::WINDOWS;
   Plan plan;
-  plan.codes_ = (unsigned *)(area + DMA64_CODES);
-  plan.prepare(lane);
+  plan.prepare(area, lane);
   iv4 wc = {0, 0, 0, 0};
   if (nl > 0) wc = *(const iv4 *)row;
   auto idx_at = [&](int k) -> int {
@@ -1355,7 +1301,7 @@ __device__ __forceinline__ void dma_list_walk(const Rec64 *__restrict__ rec, uns
   };
   // word 3: the untyped 32-bit data format the load-to-LDS path needs on gfx9-family parts
   const __amdgpu_buffer_rsrc_t rsrc =
-      __builtin_amdgcn_make_buffer_rsrc((void *)rec, (short)sizeof(Rec64), (int)n_rec, 0x00020000);
+      __builtin_amdgcn_make_buffer_rsrc((void *)rec, (short)sizeof(Rec), (int)n_rec, 0x00020000);
   auto issue = [&](int k) {
     const int j = idx_at(k);
     int jm[NI];
@@ -1363,7 +1309,7 @@ __device__ __forceinline__ void dma_list_walk(const Rec64 *__restrict__ rec, uns
     for (int q = 0; q < NI; q++) jm[q] = __builtin_amdgcn_ds_bpermute(plan.src_addr(q), j);
 #pragma unroll
     for (int q = 0; q < NI; q++)
-      __builtin_amdgcn_struct_ptr_buffer_load_lds(rsrc, (lds_void *)(area + DMA_BLOCK * q), 16,
+      __builtin_amdgcn_struct_ptr_buffer_load_lds(rsrc, (lds_void *)(area + Plan::BLOCK * q), 16,
                                                   jm[q], plan.off(q), 0, 0, 0);
   };
   if (nmax > 0) issue(0);
@@ -1392,6 +1338,104 @@ __device__ __forceinline__ void dma_list_walk(const Rec64 *__restrict__ rec, uns
   }
   __builtin_amdgcn_s_waitcnt(0);
   wave_sync_lds();  // the area is free for the next walk
+}
+
+// FP32 list pass with the cooperative gather for the main sum: the 64 lanes of a wavefront run
+// the pair loop together (inactive lanes with an empty list) because every lane helps load.
+// Loaded lane by lane, every 16-byte piece of every record would be a separate L1 access; here
+// each load instruction reads whole records from contiguous lines straight into LDS (Plan32),
+// and each lane then reads its own record from there.
+template <class Acc, int BS = BSP>
+__device__ __forceinline__ void pair32_coop(unsigned b, unsigned nb, unsigned n_init,
+                                            const State &st, const Rec32 *__restrict__ rec,
+                                            const int *__restrict__ nbr, int stride,
+                                            const int *__restrict__ count, const Consts &c,
+                                            const Counters *cnt, double *__restrict__ sums,
+                                            size_t sstride, const int *__restrict__ trav,
+                                            const double4 *__restrict__ cs4) {
+  const unsigned t = xcd_map(b, nb) * BS + threadIdx.x;
+  const unsigned N = cnt->N;
+  const bool in_range = t < N && t < n_init;
+  const unsigned i = in_range && trav ? (unsigned)trav[t] : t;
+  const uint32_t fl = in_range ? st.flags[i] : FL_BLANKED;
+  const bool act = in_range && !(fl & FL_BLANKED);
+  const unsigned ic = act ? i : 0u;
+  Pair32<Acc> P(c, rec[ic], fl);
+  const int nl = act ? count[i] : 0;
+  const int *row = nbr + (size_t)ic * stride;
+  if (c.cspm) {
+    if (P.cspm_sum())
+      gather_loop(row, nl, [&](int j) { return cs4[j]; },
+                  [&](const double4 &g) { P.cspm_term(c, g); });
+    P.cspm_end();
+  }
+  // The record gather: straight into LDS (Plan32) with Kahan sums, through registers
+  // (CoopGather) with wide sums, the faster of the two for each on the 1M single-grain cut
+  // (Kahan 1.51 -> 1.39 ms with the direct loads; wide 1.39 with registers, 1.44 direct).
+  if constexpr (!std::is_same<Acc, AccWide>::value) {
+    __shared__ __align__(16) char s_area[BS / 64][Plan32::AREA];
+    // the wave index as a scalar: the scalar unit forms the LDS destinations of the record loads
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int nmax = nl;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o, 64));
+    dma_list_walk<Plan32>(rec, n_init, row, nl, nmax, ic, lane, s_area[w],
+                          [&](const auto &pc) { P.term(c, __builtin_bit_cast(Rec32, pc)); });
+  } else {
+    typedef CoopGather<sizeof(Rec32) / 16, 8, sizeof(Rec32)> CG;
+    __shared__ __align__(16) char s_rec[2][BS / 64][64 * sizeof(Rec32)];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int nmax = nl;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o, 64));
+    typedef int iv4 __attribute__((ext_vector_type(4)));
+    iv4 win = {0, 0, 0, 0};
+    // neighbour of this lane at step k (its own particle past the end of its list); k ascends
+    auto idx_at = [&](int k) -> int {
+      if ((k & 3) == 0 && k < nl) win = *(const iv4 *)(row + k);
+      if (k >= nl) return (int)ic;
+      const int q = k & 3;
+      return q == 0 ? win.x : q == 1 ? win.y : q == 2 ? win.z : win.w;
+    };
+    piece16 v[8];
+    if (nmax > 0) {
+      CG::load((const char *)rec, sizeof(Rec32), idx_at(0), lane, v);
+      CG::store(s_rec[0][w], lane, v);
+      wave_sync_lds();
+    }
+    for (int k = 0; k < nmax; k++) {
+      const bool more = k + 1 < nmax;
+      if (more) CG::load((const char *)rec, sizeof(Rec32), idx_at(k + 1), lane, v);
+      if (k < nl) {
+        // the record as seven 16-byte reads: at the 112-byte slot stride the 8 lanes a 16-byte
+        // read serves per cycle fall on distinct banks, while 4- and 8-byte field reads would put
+        // 4 lanes on each bank
+        struct Pieces {
+          piece16 p[sizeof(Rec32) / 16];
+        } pc;
+        const char *src = s_rec[k & 1][w] + lane * sizeof(Rec32);
+#pragma unroll
+        for (int q = 0; q < (int)(sizeof(Rec32) / 16); q++)
+          pc.p[q] = *(const piece16 *)(src + 16 * q);
+        P.term(c, __builtin_bit_cast(Rec32, pc));
+      }
+      if (more) CG::store(s_rec[(k + 1) & 1][w], lane, v);
+      wave_sync_lds();
+    }
+  }
+  if (!act) return;
+  P.store(sums, sstride, i);
+}
+// BS: threads per workgroup (64, 128 or 256; each wavefront works alone, so the sums do not
+// depend on it).  Small instances spread over more compute units with smaller workgroups.
+template <class Acc, int BS>
+__global__ void __launch_bounds__(BS, PAIR32_WAVES)
+    k_pair32_coop(unsigned n_init, State st, const Rec32 *__restrict__ rec,
+                  const int *__restrict__ nbr, int stride, const int *__restrict__ count,
+                  Consts c, const Counters *cnt, double *__restrict__ sums, size_t sstride,
+                  const int *__restrict__ trav, const double4 *__restrict__ cs4) {
+  pair32_coop<Acc, BS>(blockIdx.x, gridDim.x, n_init, st, rec, nbr, stride, count, c, cnt, sums,
+                       sstride, trav, cs4);
 }
 
 template <int BS = BSP>
