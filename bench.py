@@ -110,7 +110,7 @@ PAIR_BYTES_PER_PARTICLE = {"fp64": 432.0, "fp32": 352.0}
 # committed rocprofv3 FETCH_SIZE / WRITE_SIZE summaries: (kernel, preset, scale) -> file
 TRAFFIC_PROFILES = {
     ("pair_fp64", 2, 1.0): "pair_fp64_cut1M_single_grain_traffic.md",
-    ("pair_fp32_wide", 2, 1.0): "pair_fp32_wide_cut1M_single_grain_traffic.md",
+    ("pair_fp32_kahan", 2, 1.0): "pair_fp32_kahan_cut1M_single_grain_traffic.md",
 }
 
 
