@@ -1340,6 +1340,78 @@ __device__ __forceinline__ void dma_list_walk(const typename Plan::Rec *__restri
   wave_sync_lds();  // the area is free for the next walk
 }
 
+// The CSPM walk through LDS: step k gives each lane with k < nl its neighbour k's 32-byte entry
+// (x, y, z, volume or the tool pattern).  A step's 64 entries are two load instructions straight
+// into LDS (instruction q fetches 16-byte piece g = 64 q + lane of the step's 2 KB image: entry
+// g / 2), through a buffer descriptor of 32-byte stride as in dma_list_walk.  The term is light,
+// so the walk runs in blocks of four steps with the next block's eight loads in flight while a
+// block computes: a ring of two blocks, 16 KB of the area.  The two pieces of an entry swap
+// slots every eight entries, so that the 16-byte reads of 16 consecutive lanes fall on distinct
+// bank groups.  The indices come as each lane's int4 row windows, one block ahead.
+constexpr int CSPM_STEP = 64 * 32;
+constexpr int CSPM_RING = 2 * 4 * CSPM_STEP;
+template <class Term>
+__device__ __forceinline__ void dma_cspm_walk(const double4 *__restrict__ cs4, unsigned n_rec,
+                                              const int *row, int nl, int nmax, unsigned ic,
+                                              int lane, char *area, Term &&term) {
+  typedef int iv4 __attribute__((ext_vector_type(4)));
+  constexpr int S = 4;  // steps per block: one int4 window of indices
+  const int nb = (nmax + S - 1) / S;
+  if (nb == 0) return;
+  int src[2], off[2];
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int g = 64 * q + lane, r = g >> 1;
+    src[q] = 4 * r;
+    off[q] = 16 * (((g & 1) + (r >> 3)) & 1);
+  }
+  const int rd0 = 32 * lane + 16 * ((lane >> 3) & 1);
+  const int rd1 = 32 * lane + 16 * (((lane >> 3) + 1) & 1);
+  const __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void *)cs4, (short)sizeof(double4), (int)n_rec, 0x00020000);
+  // the indices of block b (the lane's own record past its list)
+  auto window = [&](int b) -> iv4 {
+    const int k = S * b;
+    iv4 w = {(int)ic, (int)ic, (int)ic, (int)ic};
+    if (k < nl) w = *(const iv4 *)(row + k);
+    return w;
+  };
+  auto issue = [&](int b, const iv4 &w) {
+    char *base = area + (b & 1) * S * CSPM_STEP;
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const int j = S * b + s < nl ? w[s] : (int)ic;
+#pragma unroll
+      for (int q = 0; q < 2; q++)
+        __builtin_amdgcn_struct_ptr_buffer_load_lds(
+            rsrc, (lds_void *)(base + s * CSPM_STEP + 1024 * q), 16,
+            __builtin_amdgcn_ds_bpermute(src[q], j), off[q], 0, 0, 0);
+    }
+  };
+  issue(0, window(0));
+  iv4 wn = nb > 1 ? window(1) : iv4{0, 0, 0, 0};
+  for (int b = 0; b < nb; b++) {
+    __builtin_amdgcn_s_waitcnt(0);  // block b's entries (and the next window) have arrived
+    wave_sync_lds();
+    if (b + 1 < nb) issue(b + 1, wn);  // into the half block b - 1 used, read before
+    if (b + 2 < nb) wn = window(b + 2);
+    const char *base = area + (b & 1) * S * CSPM_STEP;
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      if (S * b + s < nl) {
+        struct Pieces {
+          piece16 p[2];
+        } pc;
+        pc.p[0] = *(const piece16 *)(base + s * CSPM_STEP + rd0);
+        pc.p[1] = *(const piece16 *)(base + s * CSPM_STEP + rd1);
+        term(__builtin_bit_cast(double4, pc));
+      }
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  wave_sync_lds();  // the area is free for the next walk
+}
+
 // FP32 list pass with the cooperative gather for the main sum: the 64 lanes of a wavefront run
 // the pair loop together (inactive lanes with an empty list) because every lane helps load.
 // Loaded lane by lane, every 16-byte piece of every record would be a separate L1 access; here
@@ -1363,25 +1435,36 @@ __device__ __forceinline__ void pair32_coop(unsigned b, unsigned nb, unsigned n_
   Pair32<Acc> P(c, rec[ic], fl);
   const int nl = act ? count[i] : 0;
   const int *row = nbr + (size_t)ic * stride;
-  if (c.cspm) {
-    if (P.cspm_sum())
-      gather_loop(row, nl, [&](int j) { return cs4[j]; },
-                  [&](const double4 &g) { P.cspm_term(c, g); });
-    P.cspm_end();
-  }
-  // The record gather: straight into LDS (Plan32) with Kahan sums, through registers
-  // (CoopGather) with wide sums, the faster of the two for each on the 1M single-grain cut
-  // (Kahan 1.51 -> 1.39 ms with the direct loads; wide 1.39 with registers, 1.44 direct).
+  // The gathers: straight into LDS (dma_cspm_walk, Plan32) with Kahan sums, per lane and through
+  // registers (CoopGather) with wide sums, the faster of the two for each on the 1M single-grain
+  // cut (Kahan 1.51 -> 1.39 -> 1.29 ms with the direct record, then CSPM-entry loads; wide 1.39
+  // with registers, 1.44 with direct record loads, 1.95 with direct CSPM-entry loads).
   if constexpr (!std::is_same<Acc, AccWide>::value) {
-    __shared__ __align__(16) char s_area[BS / 64][Plan32::AREA];
-    // the wave index as a scalar: the scalar unit forms the LDS destinations of the record loads
+    constexpr int AREA = Plan32::AREA > CSPM_RING ? Plan32::AREA : CSPM_RING;
+    __shared__ __align__(16) char s_area[BS / 64][AREA];
+    // the wave index as a scalar: the scalar unit forms the LDS destinations of the loads
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (c.cspm) {
+      const int nc = P.cspm_sum() ? nl : 0;
+      int ncmax = nc;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) ncmax = max(ncmax, __shfl_xor(ncmax, o, 64));
+      dma_cspm_walk(cs4, n_init, row, nc, ncmax, ic, lane, s_area[w],
+                    [&](const double4 &g) { P.cspm_term(c, g); });
+      P.cspm_end();
+    }
     int nmax = nl;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o, 64));
     dma_list_walk<Plan32>(rec, n_init, row, nl, nmax, ic, lane, s_area[w],
                           [&](const auto &pc) { P.term(c, __builtin_bit_cast(Rec32, pc)); });
   } else {
+    if (c.cspm) {
+      if (P.cspm_sum())
+        gather_loop(row, nl, [&](int j) { return cs4[j]; },
+                    [&](const double4 &g) { P.cspm_term(c, g); });
+      P.cspm_end();
+    }
     typedef CoopGather<sizeof(Rec32) / 16, 8, sizeof(Rec32)> CG;
     __shared__ __align__(16) char s_rec[2][BS / 64][64 * sizeof(Rec32)];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1464,8 +1547,12 @@ __device__ __forceinline__ void pair64_dma(unsigned b, unsigned nb, unsigned n_i
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   char *area = s_area[w];
   if (c.cspm) {
-    if (P.cspm_sum())
-      gather_loop(row, nl, [&](int j) { return cs4[j]; },
+    const int nc = P.cspm_sum() ? nl : 0;
+    int ncmax = nc;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) ncmax = max(ncmax, __shfl_xor(ncmax, o, 64));
+    static_assert(CSPM_RING <= DMA64_AREA, "the CSPM ring fits the main walk's area");
+    dma_cspm_walk(cs4, n_init, row, nc, ncmax, ic, lane, area,
                   [&](const double4 &g) { P.cspm_term(c, g); });
     P.cspm_end();
   }
