@@ -393,6 +393,11 @@ int mfg_leap_frog_reset_contact_forces(mfg_leap_frog *lf, mfg_particles *p);
 int mfg_debug_check_valid(mfg_particles *p);
 
 /* ---- per-kernel timing (HIP events on the library stream) ----------------------------------- */
+/* on: 0 off; MFG_PROFILE_ALL a pair of events around every kernel; MFG_PROFILE_PASSES around the
+ * neighbour and pair passes only (each event pair adds a little to a step: a timed run of many
+ * steps records only what it needs) */
+#define MFG_PROFILE_ALL 1
+#define MFG_PROFILE_PASSES 2
 int mfg_profile_enable(int on);
 int mfg_profile_reset(void);
 /* number of kernels recorded; for k < n: name (<= 47 chars), launches and total milliseconds */

@@ -19,6 +19,7 @@ MFG_FP64, MFG_FP32_KAHAN, MFG_FP32_WIDE = 0, 1, 2
 MFG_HEAT_NONE, MFG_HEAT_BROOKSHAW, MFG_HEAT_PSE = 0, 1, 2
 MFG_CONTACT_EXHAUSTIVE, MFG_CONTACT_HASHED = 0, 1
 MFG_GRID_GREEN, MFG_GRID_ROTHLIN = 0, 1
+MFG_PROFILE_ALL, MFG_PROFILE_PASSES = 1, 2
 MFG_TOOL_WEDGE, MFG_TOOL_GRAIN = 0, 1
 
 F_POS, F_VEL, F_VEL_BC, F_FC, F_FT, F_H, F_RHO, F_P, F_S, F_R = range(10)

@@ -721,7 +721,9 @@ class batch:
 
 
 # ---- profiling ---------------------------------------------------------------------------------
-def profile_enable(on: bool = True) -> None:
+def profile_enable(on: int = True) -> None:
+    """0 / False: off; True / MFG_PROFILE_ALL: events around every kernel; MFG_PROFILE_PASSES:
+    around the neighbour and pair passes only."""
     check(_L().mfg_profile_enable(int(on)))
 
 

@@ -57,8 +57,9 @@ struct Lib {
   mfg_joco_constants joco{};
   mfg_trml_constants a_twp{}, a_ttool{}, i_twp{}, i_ttool{};
   bool plastic = false, a_thermal = false, a_fric = false, i_thermal_wp = false;
-  // profiling
-  bool prof = false;
+  // profiling: 0 off, MFG_PROFILE_ALL every kernel, MFG_PROFILE_PASSES the neighbour and pair
+  // passes only
+  int prof = 0;
   std::vector<hipEvent_t> pool;
   struct Pending {
     std::string name;
@@ -119,6 +120,9 @@ struct ProfScope {
   std::string name;
   ProfScope(const char *n) {
     if (!g_lib.prof) return;
+    if (g_lib.prof == MFG_PROFILE_PASSES && std::strcmp(n, "neighbours") != 0 &&
+        std::strncmp(n, "pair_fp", 7) != 0)
+      return;
     name = n;
     hipEvent_t a = prof_event();
     b = prof_event();
@@ -1787,9 +1791,9 @@ struct StepWork {
 
 // suspends the per-kernel event timing for the scope (its events are created on demand)
 struct ProfOff {
-  bool saved;
+  int saved;
   ProfOff(bool on) : saved(g_lib.prof) {
-    if (on) g_lib.prof = false;
+    if (on) g_lib.prof = 0;
   }
   ~ProfOff() { g_lib.prof = saved; }
 };
@@ -2533,8 +2537,9 @@ int mfg_batch_step(mfg_batch *b, int n_steps, int record_forces, int use_graph) 
 // ---- profiling ----
 int mfg_profile_enable(int on) {
   NEED_LIB();
+  if (on < 0 || on > MFG_PROFILE_PASSES) return fail(MFG_ERR_ARG, "profile level %d", on);
   if (!on) prof_collect();
-  g_lib.prof = on != 0;
+  g_lib.prof = on;
   return MFG_OK;
 }
 int mfg_profile_reset(void) {
