@@ -213,9 +213,12 @@ int mfg_particle_gpu_snapshot(mfg_particles *p, void *host);
 int mfg_particle_gpu_restore(mfg_particles *p, const void *host);
 int mfg_field_width(int field);
 /* work of the last neighbour pass: candidate pairs visited and interacting pairs kept
- * (synchronises); no counterpart in the reference, used for the roofline accounting */
+ * (synchronises; MFG_ERR_STATE if that pass did not count them); no counterpart in the
+ * reference, used for the roofline accounting.  Counting is off by default (it adds about 3 %
+ * to the neighbour pass) and switched for every later pass by mfg_pair_stats_enable. */
 int mfg_particle_gpu_pair_stats(mfg_particles *p, unsigned long long *candidates,
                                 unsigned long long *pairs);
+int mfg_pair_stats_enable(int on);
 
 /* ---- grid_base / grid_gpu_green / grid_gpu_rothlin (src/grid_gpu_base.h:82-114) ------------ */
 /* locked geometry: grid_gpu_green(int num_part, float3_t bbmin, float3_t bbmax, float_t h),

@@ -137,6 +137,7 @@ SIGNATURES = {
     "mfg_particle_gpu_restore": (_i, [_vp, _vp]),
     "mfg_field_width": (_i, [_i]),
     "mfg_particle_gpu_pair_stats": (_i, [_vp, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]),
+    "mfg_pair_stats_enable": (_i, [_i]),
     "mfg_grid_create_locked": (_i, [_i, _i, _dp, _dp, _d, C.POINTER(_vp)]),
     "mfg_grid_create_unlocked": (_i, [_i, C.c_long, _i, C.POINTER(_vp)]),
     "mfg_grid_destroy": (_i, [_vp]),

@@ -202,7 +202,8 @@ class particle_gpu:
         return n.value, ni.value
 
     def pair_stats(self):
-        """(candidate pairs, interacting pairs) of the last neighbour pass."""
+        """(candidate pairs, interacting pairs) of the last neighbour pass, which must have
+        counted them (pair_stats_enable)."""
         a, b = C.c_ulonglong(0), C.c_ulonglong(0)
         check(_L().mfg_particle_gpu_pair_stats(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
@@ -721,6 +722,12 @@ class batch:
 
 
 # ---- profiling ---------------------------------------------------------------------------------
+def pair_stats_enable(on: bool = True) -> None:
+    """Whether later neighbour passes count their candidate and interacting pairs (read with
+    particle_gpu.pair_stats); off by default."""
+    check(_L().mfg_pair_stats_enable(int(bool(on))))
+
+
 def profile_enable(on: int = True) -> None:
     """0 / False: off; True / MFG_PROFILE_ALL: events around every kernel; MFG_PROFILE_PASSES:
     around the neighbour and pair passes only."""

@@ -60,6 +60,8 @@ struct Lib {
   // profiling: 0 off, MFG_PROFILE_ALL every kernel, MFG_PROFILE_PASSES the neighbour and pair
   // passes only
   int prof = 0;
+  // whether the neighbour passes count their candidate and interacting pairs
+  bool count_pairs = false;
   std::vector<hipEvent_t> pool;
   struct Pending {
     std::string name;
@@ -518,10 +520,11 @@ int contact_search_buffers(mfg_particles *p, int n_tools, ContactSearch &cs) {
 
 // neighbour lists (+ PSE) of the current records on grid g; stats: keep the per-workgroup
 // candidate / pair totals (a buffer sized on first use) for mfg_particle_gpu_pair_stats
+// stats: the pass may count its pairs (when counting is on, mfg_pair_stats_enable)
 int run_neighbours(mfg_grid *g, mfg_particles *p, const Consts &c, int want_pse, bool stats = true) {
   const GridInfo gi = g->info();
   const int tile = resolve_cell_tile(g_lib.opt.cell_tile, gi.num_cell, p->N_init);
-  if (!stats) {
+  if (!stats || !g_lib.count_pairs) {
     p->stat_n = 0;
     launch_neighbours(g_lib.stream, gi, g->count, p->nl4, p->pse2, p->st().h, p->st().flags, c,
                       p->nbr, p->nl_stride, p->count, p->st().T_t, p->cnt, want_pse, nullptr,
@@ -984,6 +987,8 @@ int mfg_particle_gpu_pair_stats(mfg_particles *p, unsigned long long *cand,
   NEED_LIB();
   if (!p) return fail(MFG_ERR_ARG, "null particles");
   int rc = check_counters(p);
+  if (rc == MFG_OK && p->stat_n == 0)
+    rc = fail(MFG_ERR_STATE, "the last neighbour pass did not count its pairs (mfg_pair_stats_enable)");
   unsigned long long a = 0, b = 0;
   if (p->stat_n > 0) {
     std::vector<unsigned long long> h(2 * (size_t)p->stat_n);
@@ -2535,6 +2540,11 @@ int mfg_batch_step(mfg_batch *b, int n_steps, int record_forces, int use_graph) 
 
 
 // ---- profiling ----
+int mfg_pair_stats_enable(int on) {
+  NEED_LIB();
+  g_lib.count_pairs = on != 0;
+  return MFG_OK;
+}
 int mfg_profile_enable(int on) {
   NEED_LIB();
   if (on < 0 || on > MFG_PROFILE_PASSES) return fail(MFG_ERR_ARG, "profile level %d", on);
