@@ -246,7 +246,10 @@ struct mfg_grid {
   int nparts_cap = 0;
   double *bbox_part = nullptr;
   std::vector<double> bbox_host;  // host copy of the bbox partials (sized at creation)
-  bool scanned = false;  // offsets valid for the last hashed particle set
+  // keys, offsets and sort permutation made by a hash (mfg_grid_assign_hashes) for sorted_for
+  // and not yet applied; a step or a sort applies them
+  bool scanned = false;
+  const mfg_particles *sorted_for = nullptr;  // the particle set the last sort was made for
   GridInfo info() const {
     GridInfo g;
     g.nx = (int)nx; g.ny = (int)ny; g.nz = (int)nz;
@@ -398,11 +401,12 @@ void grid_geometry(const mfg_grid *g, double geom[7]) {
 }
 
 // Whether a step may keep the particle order and cell tables of the last sort (the opt-in
-// re-sort interval): the interval is not over, and the set was last sorted on this grid, which is
-// locked, does not move and has the geometry it had then.
+// re-sort interval): the interval is not over, and the set was last sorted on this grid, which
+// has sorted no other set since, is locked, does not move and has the geometry it had then.
 bool keep_last_sort(const mfg_grid *g, const mfg_particles *p) {
   const int k = g_lib.opt.sort_interval;
-  if (k <= 1 || p->sorted_grid != g || p->since_sort + 1 >= k || !g->locked) return false;
+  if (k <= 1 || p->sorted_grid != g || g->sorted_for != p || p->since_sort + 1 >= k || !g->locked)
+    return false;
   if (g->vel_bbox[0] != 0. || g->vel_bbox[1] != 0. || g->vel_bbox[2] != 0.) return false;
   double geom[7];
   grid_geometry(g, geom);
@@ -441,6 +445,7 @@ void sort_keys(mfg_grid *g, mfg_particles *p, const mfg_blanking_state *b) {
     launch_fixup(g_lib.stream, gi, gb);
   }
   g->scanned = true;
+  g->sorted_for = p;
   p->sorted_grid = g;
   grid_geometry(g, p->sorted_geom);
   p->since_sort = 0;
@@ -961,7 +966,10 @@ int mfg_particle_gpu_create(unsigned N, const double *pos4, const double *vel4, 
     fu[i] = f;
     fu[n + i] = i;
   }
+  // the permutation starts as the identity (the ids): a set is never gathered through
+  // unwritten indices
   ok = ok && hipMemcpyAsync(p->uarena[0], fu.data(), sizeof(uint32_t) * 2 * n, hipMemcpyHostToDevice, s) == hipSuccess &&
+       hipMemcpyAsync(p->order, fu.data() + n, sizeof(int) * n, hipMemcpyHostToDevice, s) == hipSuccess &&
        hipStreamSynchronize(s) == hipSuccess;
   if (!ok) return cleanup(fail(MFG_ERR_HIP, "particle upload failed"));
   // counters: N = N_init until the first sort
@@ -1388,7 +1396,9 @@ int mfg_grid_adapt_particle_number(const mfg_grid *g, mfg_particles *p) {
 int mfg_grid_sort(mfg_grid *g, mfg_particles *p) {
   NEED_LIB();
   if (!g || !p) return fail(MFG_ERR_ARG, "null argument");
-  if (!g->scanned) sort_keys(g, p, nullptr);
+  // the hash of mfg_grid_assign_hashes is used only if it was made for this set (its
+  // permutation indexes this set's fields) and not yet applied
+  if (!g->scanned || g->sorted_for != p) sort_keys(g, p, nullptr);
   const int rc = check_counters(p);
   if (rc != MFG_OK) return rc;
   Consts c = make_consts(0., p->precision);
@@ -1921,10 +1931,10 @@ int step_impl(mfg_leap_frog *lf, mfg_particles *p, mfg_grid *g, int record,
     p->since_sort++;
     permute(g, p, c, 1, true);
   } else {
-    g->scanned = false;
     sort_keys(g, p, blanking);
     permute(g, p, c, 1);
   }
+  g->scanned = false;  // the permutation is applied: a later mfg_grid_sort hashes again
   // pair pass
   const bool pse = c.heat == MFG_HEAT_PSE && c.pse_on;
   {
@@ -2381,7 +2391,8 @@ void batch_mirror_step(mfg_batch::Member &m) {
   m.t = m.t + dt;
   m.p->cur ^= 1;
   m.p->stat_n = 0;
-  m.g->scanned = true;
+  m.g->scanned = false;
+  m.g->sorted_for = m.p;
 }
 
 }  // namespace

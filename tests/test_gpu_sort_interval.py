@@ -114,3 +114,30 @@ def test_allocation_free_step_with_an_interval(hip_device):
     assert a["N"] == b["N"]
     for _, _, name in FIELDS:
         np.testing.assert_array_equal(a[name], b[name], err_msg=name)
+
+
+def test_a_sort_of_another_set_on_the_grid_forces_a_sort(hip_device):
+    """With interval 4, a set's second step would keep its first sort; once another set of the
+    same size has been sorted on the same grid, whose cell tables then describe that set, the
+    step sorts again: two steps then equal two steps of the reference's schedule."""
+    def state(parts):
+        ids = parts.download(_lib.F_ID).astype(np.int64)
+        return {name: by_id(ids, parts.download(f)) for f, _, name in FIELDS}
+
+    sc = scenes.Scene(1, 0.5, 3)
+    other = scenes.Scene(1, 0.5, 4)
+    assert other.n == sc.n
+    out = {}
+    for interval in (1, 4):
+        parts, grid = scenes.setup_cut(sc, precision=_lib.MFG_FP64, sort_interval=interval)
+        api.global_time_current = sc.params.dt
+        lf = api.leap_frog(sc.n, grid.num_cell())
+        scenes.run(lf, parts, grid, 1)
+        if interval == 4:
+            b = api.particle_gpu(other.pos, other.vel, other.rho, other.T, other.h, other.fixed,
+                                 other.tool_p, precision=_lib.MFG_FP64)
+            grid.sort(b)
+        scenes.run(lf, parts, grid, 1)
+        out[interval] = state(parts)
+    for _, _, name in FIELDS:
+        np.testing.assert_array_equal(out[4][name], out[1][name], err_msg=name)
