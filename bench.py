@@ -417,6 +417,8 @@ def main():
                     help="precision form instead: run the workload --steps steps in FP64 and in both FP32 "
                          "modes and report the FP32 L2 and max errors against the FP64 run")
     args = ap.parse_args()
+    if args.steps < 1 or args.warmup < 0:
+        ap.error("--steps must be at least 1 and --warmup at least 0")
     if args.copy_probe_mib > 0:
         copy_probe(args.copy_probe_mib)
         return
